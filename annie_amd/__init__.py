@@ -1,0 +1,213 @@
+"""annie_amd -- MI355X (gfx950) engine for annie's modular-squaring loop.
+
+Python view of the C ABI in include/annie_hip.h.  `Transform` mirrors the
+reference's `transform` interface (src/transform.h:18-45) with the same method
+names, argument meaning and error behaviour: sizes outside the supported
+range raise RuntimeError (the reference throws std::runtime_error,
+src/transformGPU.h:853-857), and gcd(N, R) > 1 yields an object whose
+init_failed() is True and whose get_factor() returns the common factor
+(src/transformGPU.h:652-661).
+
+The library is the native HIP build in annie_amd/lib; importing this module
+never falls back to another implementation: a missing library raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(_HERE, "lib")
+HIP_LIB_PATH = os.path.join(LIBDIR, "libannie_hip.so")
+HOST_LIB_PATH = os.path.join(LIBDIR, "libannie_host.so")
+
+Reg0, Reg1, Reg2 = 0, 1, 2
+
+ANNIE_OK = 0
+ANNIE_INIT_FAILED = 1
+
+_lib = None
+
+
+def int_to_limbs(x):
+    """Non-negative int -> (ctypes uint64 array, limb count), little-endian limbs."""
+    if x < 0:
+        raise ValueError("negative integer")
+    n = max(1, (x.bit_length() + 63) // 64)
+    arr = (ctypes.c_uint64 * n).from_buffer_copy(x.to_bytes(8 * n, "little"))
+    return arr, n
+
+
+def limbs_to_int(arr, n):
+    """First n little-endian uint64 limbs of a ctypes array -> int."""
+    if n <= 0:
+        return 0
+    return int.from_bytes(ctypes.string_at(ctypes.addressof(arr), 8 * n), "little")
+
+
+def load_library(path=HIP_LIB_PATH):
+    """Load libannie_hip.so (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `python -m annie_amd.build`")
+    lib = ctypes.CDLL(path)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sz = ctypes.c_size_t
+    h = ctypes.c_void_p
+    lib.annie_hip_select_plan.argtypes = [u64p, sz, ctypes.c_int32, ip, ip, ip, ip]
+    lib.annie_hip_create.argtypes = [u64p, sz, ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.POINTER(h)]
+    lib.annie_hip_get_factor.argtypes = [h, u64p, sz, ctypes.POINTER(sz)]
+    lib.annie_hip_set1.argtypes = [h]
+    lib.annie_hip_copy.argtypes = [h, sz, sz]
+    lib.annie_hip_square_mul.argtypes = [h, ctypes.c_int32]
+    lib.annie_hip_square_mul_bits.argtypes = [h, ctypes.POINTER(ctypes.c_uint8), sz, ctypes.c_int32]
+    lib.annie_hip_mul.argtypes = [h, sz, sz]
+    lib.annie_hip_get.argtypes = [h, sz, u64p, sz, ctypes.POINTER(sz)]
+    lib.annie_hip_checkpoint_size.argtypes = [h]
+    lib.annie_hip_checkpoint_size.restype = sz
+    lib.annie_hip_save_checkpoint.argtypes = [h, ctypes.c_void_p, sz]
+    lib.annie_hip_load_checkpoint.argtypes = [h, ctypes.c_void_p, sz]
+    lib.annie_hip_info.argtypes = [h, ip, ip, ip, ip]
+    lib.annie_hip_sync.argtypes = [h]
+    lib.annie_hip_read_register.argtypes = [h, sz, ctypes.POINTER(ctypes.c_uint32), sz]
+    lib.annie_hip_write_register.argtypes = [h, sz, ctypes.POINTER(ctypes.c_uint32), sz]
+    lib.annie_hip_debug_square_stage.argtypes = [h, ctypes.c_int32, ctypes.c_int]
+    lib.annie_hip_read_tq.argtypes = [h, ctypes.POINTER(ctypes.c_uint32), sz]
+    lib.annie_hip_destroy.argtypes = [h]
+    lib.annie_hip_destroy.restype = None
+    lib.annie_hip_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _err(lib):
+    return lib.annie_hip_last_error().decode(errors="replace")
+
+
+def select_plan(N, a):
+    """(K, pck, LN, lbase) chosen for N and base a (src/transformGPU.h:822-850)."""
+    lib = load_library()
+    arr, n = int_to_limbs(N)
+    K, pck, LN, lb = (ctypes.c_int() for _ in range(4))
+    rc = lib.annie_hip_select_plan(arr, n, a, ctypes.byref(K), ctypes.byref(pck), ctypes.byref(LN), ctypes.byref(lb))
+    if rc != ANNIE_OK:
+        raise RuntimeError(_err(lib))
+    return K.value, pck.value, LN.value, lb.value
+
+
+class Transform:
+    """MMGW register machine for one N on one GPU (src/transform.h:18-45)."""
+
+    Reg0, Reg1, Reg2 = Reg0, Reg1, Reg2
+
+    def __init__(self, N, a, device=0, min_ln=10):
+        lib = load_library()
+        self._lib = lib
+        self._h = ctypes.c_void_p()
+        self.N = int(N)
+        arr, n = int_to_limbs(self.N)
+        rc = lib.annie_hip_create(arr, n, int(a), int(device), int(min_ln), ctypes.byref(self._h))
+        if rc < 0:
+            self._h = ctypes.c_void_p()
+            raise RuntimeError(_err(lib))
+        self._init_failed = rc == ANNIE_INIT_FAILED
+        K, pck, LN, lb = (ctypes.c_int() for _ in range(4))
+        lib.annie_hip_info(self._h, ctypes.byref(K), ctypes.byref(pck), ctypes.byref(LN), ctypes.byref(lb))
+        self.K, self.pck, self.LN, self.lbase = K.value, pck.value, LN.value, lb.value
+        self.n = 1 << self.LN
+
+    def _check(self, rc):
+        if rc != ANNIE_OK:
+            raise RuntimeError(_err(self._lib))
+
+    def init_failed(self):
+        return self._init_failed
+
+    def get_factor(self):
+        cap = (self.N.bit_length() + 63) // 64 + 1
+        out = (ctypes.c_uint64 * cap)()
+        nout = ctypes.c_size_t()
+        self._check(self._lib.annie_hip_get_factor(self._h, out, cap, ctypes.byref(nout)))
+        return limbs_to_int(out, nout.value)
+
+    def set1(self):
+        self._check(self._lib.annie_hip_set1(self._h))
+
+    def copy(self, dst, src):
+        self._check(self._lib.annie_hip_copy(self._h, dst, src))
+
+    def square_mul(self, a):
+        self._check(self._lib.annie_hip_square_mul(self._h, int(a)))
+
+    def square_mul_bits(self, bits, a):
+        """r0 = (bit ? a : 1) * r0^2 for each bit of `bits` (bytes or list of 0/1)."""
+        buf = (ctypes.c_uint8 * len(bits))(*bits)
+        self._check(self._lib.annie_hip_square_mul_bits(self._h, buf, len(bits), int(a)))
+
+    def mul(self, dst, src):
+        self._check(self._lib.annie_hip_mul(self._h, dst, src))
+
+    def get(self, src):
+        cap = (self.N.bit_length() + 63) // 64 + 1
+        out = (ctypes.c_uint64 * cap)()
+        nout = ctypes.c_size_t()
+        self._check(self._lib.annie_hip_get(self._h, src, out, cap, ctypes.byref(nout)))
+        return limbs_to_int(out, nout.value)
+
+    def save_checkpoint(self):
+        size = self._lib.annie_hip_checkpoint_size(self._h)
+        buf = ctypes.create_string_buffer(size)
+        self._check(self._lib.annie_hip_save_checkpoint(self._h, buf, size))
+        return buf.raw
+
+    def read_checkpoint(self, data):
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        rc = self._lib.annie_hip_load_checkpoint(self._h, buf, len(data))
+        return rc == ANNIE_OK
+
+    def has_checkpoint(self):
+        return True
+
+    def sync(self):
+        self._check(self._lib.annie_hip_sync(self._h))
+
+    def read_register(self, reg):
+        """Raw residues of a device register (K planes of n words, plane-major)."""
+        words = self.K * self.n
+        out = (ctypes.c_uint32 * words)()
+        self._check(self._lib.annie_hip_read_register(self._h, reg, out, words))
+        return out
+
+    def write_register(self, reg, words):
+        buf = (ctypes.c_uint32 * len(words))(*words)
+        self._check(self._lib.annie_hip_write_register(self._h, reg, buf, len(words)))
+
+    def debug_square_stage(self, a, stage):
+        self._check(self._lib.annie_hip_debug_square_stage(self._h, int(a), int(stage)))
+
+    def read_tq(self):
+        words = 3 * self.n
+        out = (ctypes.c_uint32 * words)()
+        self._check(self._lib.annie_hip_read_tq(self._h, out, words))
+        return out
+
+    def get_info_short(self):
+        return f"{self.K}-prime NTT of length {self.n}, packed-base = 2^{{{self.pck}x{self.lbase}}}"
+
+    def close(self):
+        if self._h:
+            self._lib.annie_hip_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,63 @@
+// Kernel argument structures shared by the gfx950 kernels and the host engine.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "arith.h"
+
+namespace annie {
+
+struct TwDev {
+  uint32_t w, wq;
+};
+
+// Elements per thread in the NTT passes (log2 = levels per register round).
+constexpr int NTT_R = 16;
+constexpr int NTT_LOGR = 4;
+// Consecutive coefficients per thread in the carry kernel.
+constexpr int CARRY_L = 4;
+constexpr int CARRY_WG = 256;
+// 32-bit limbs of a packed digit (coefficients carry at most 72 bits plus sign).
+constexpr int DIGIT_LIMBS = 3;
+
+// One transform job: K residue planes (plane stride n words).
+struct NttJob {
+  const uint32_t* src;  // input planes
+  uint32_t* dst;        // output planes (may equal src)
+  const TwDev* fwd;     // forward twiddles of prime 0 (prime k at + k * n)
+  const TwDev* inv;     // inverse twiddles of prime 0
+  const uint32_t* opnd; // pointwise operand planes (MULC / MULV), else null
+};
+
+struct NttArgs {
+  NttJob job[2];
+  uint32_t p[KMAX], q[KMAX];
+  uint32_t sc[KMAX], scq[KMAX];  // n^-1 * 2^64 (Montgomery scale of a data product)
+  int LN;        // log2 of the transform length
+  int m;         // log2 of the line length handled by this pass
+  int m1;        // levels done by the column pass (0 for a single-pass transform)
+  int cols;      // columns per workgroup (column passes) / lines per workgroup (row pass)
+};
+
+enum RowMode : int { ROW_FWD = 0, ROW_SQR = 1, ROW_MULC = 2, ROW_MULV = 3 };
+
+struct CarryJob {
+  const uint32_t* src;  // K residue planes of the convolution (exact values mod p_k)
+  uint32_t* dst;        // K residue planes, or DIGIT_LIMBS raw digit planes if raw_out
+  const uint32_t* sub;  // DIGIT_LIMBS raw digit planes subtracted from each coefficient, or null
+  uint32_t a;           // multiplier of every coefficient
+  int nega;             // 1: value mod B^n + 1 (wrap negated), 0: mod B^n - 1
+  int canon;            // 0: none, 1: reduce into [0, R) then add 2, 2: reduce into [0, Q)
+  int raw_out;          // 1: write raw digits instead of residues
+};
+
+struct CarryArgs {
+  CarryJob job[2];
+  uint32_t p[KMAX], q[KMAX];
+  uint32_t c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX];
+  uint32_t ginv[KMAX][KMAX], ginvq[KMAX][KMAX];
+  int LN;
+  int cbits;  // bits per coefficient (pck * lbase)
+};
+
+}  // namespace annie

@@ -1,0 +1,106 @@
+/*
+ * annie_hip.h -- C ABI of the MI355X (gfx950) modular-squaring engine.
+ *
+ * One context holds one modulus N and the register machine of the reference's
+ * `transform` interface (src/transform.h:18-45): three user registers
+ * Reg0..Reg2 holding values in Montgomery-McLaughlin-Gallot-Woltman form
+ * x* = 2^-1 R x mod N (doc/annie.tex:138-164).  Each entry point below names
+ * the reference interface it replaces.  Plain pointers and sizes only; no
+ * exception crosses the ABI: every call returns a status and
+ * annie_hip_last_error() describes the last failure of the calling thread.
+ * A context is used by one host thread at a time.  Big integers are passed as
+ * little-endian arrays of 64-bit limbs.
+ */
+#ifndef ANNIE_HIP_H
+#define ANNIE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct annie_hip annie_hip_t;
+
+enum {
+  ANNIE_OK = 0,
+  ANNIE_INIT_FAILED = 1,   /* gcd(N, R) > 1: the factor is available (reference: init_failed()) */
+  ANNIE_ERR = -1,          /* HIP runtime failure */
+  ANNIE_ERR_SIZE = -2,     /* transform size outside the supported range */
+  ANNIE_ERR_ARG = -3,      /* invalid argument (register index, buffer too small, odd/even N ...) */
+  ANNIE_ERR_NOMEM = -4     /* device allocation failed */
+};
+
+/* Register indices (src/transform.h:21). */
+enum { ANNIE_REG0 = 0, ANNIE_REG1 = 1, ANNIE_REG2 = 2 };
+
+/* Transform size chosen for N and base a, exactly as create_transformGPU
+ * (src/transformGPU.h:822-850).  No GPU needed. */
+int annie_hip_select_plan(const uint64_t* N, size_t nlimbs, int32_t a, int* K, int* pck, int* LN, int* lbase);
+
+/* Replaces create_transformGPU(N, a, device) (src/transformGPU.h:822-885) and the
+ * transformGPU constructor (:644-753).  min_ln is the smallest accepted log2
+ * transform length (the reference accepts 10..23, or 4..23 in its QUICK_TEST
+ * build, src/transformGPU.h:852-857).  Returns ANNIE_INIT_FAILED with a
+ * context whose factor can be read when gcd(N, R) > 1. */
+int annie_hip_create(const uint64_t* N, size_t nlimbs, int32_t a, int device, int min_ln, annie_hip_t** out);
+
+/* transform::get_factor (src/transform.h:31; src/transformGPU.h:604). */
+int annie_hip_get_factor(const annie_hip_t* ctx, uint64_t* limbs, size_t cap, size_t* nout);
+
+/* transform::set1 -- r0 = 1 (src/transform.h:25; src/transformGPU.h:571-574). */
+int annie_hip_set1(annie_hip_t* ctx);
+
+/* transform::copy -- r_dst = r_src (src/transform.h:27; src/transformGPU.h:588-591). */
+int annie_hip_copy(annie_hip_t* ctx, size_t dst, size_t src);
+
+/* transform::square_mul -- r0 = a * r0^2 (src/transform.h:28; src/transformGPU.h:593-596). */
+int annie_hip_square_mul(annie_hip_t* ctx, int32_t a);
+
+/* The reference's hot loop body for `count` exponent bits (src/annie.h:319-341 without
+ * the Gerbicz-Li products): for i < count, r0 = (bits[i] ? a : 1) * r0^2. */
+int annie_hip_square_mul_bits(annie_hip_t* ctx, const uint8_t* bits, size_t count, int32_t a);
+
+/* transform::mul -- r_dst *= r_src (src/transform.h:29; src/transformGPU.h:598-601). */
+int annie_hip_mul(annie_hip_t* ctx, size_t dst, size_t src);
+
+/* transform::get -- canonical value of r_src in [0, N) (src/transform.h:26;
+ * src/transformGPU.h:576-586).  Writes up to cap limbs, *nout = limbs used. */
+int annie_hip_get(annie_hip_t* ctx, size_t src, uint64_t* limbs, size_t cap, size_t* nout);
+
+/* transform::save_checkpoint / read_checkpoint payload for r0 and r1
+ * (src/transformGPU.h:606-641): u32 K, u32 n, then for each prime k the plane
+ * of r0 followed by the plane of r1 (n u32 words each). */
+size_t annie_hip_checkpoint_size(const annie_hip_t* ctx);
+int annie_hip_save_checkpoint(annie_hip_t* ctx, void* buf, size_t cap);
+int annie_hip_load_checkpoint(annie_hip_t* ctx, const void* buf, size_t size);
+
+/* get_info_short fields (src/transformGPU.h:784-789). */
+int annie_hip_info(const annie_hip_t* ctx, int* K, int* pck, int* LN, int* lbase);
+
+/* Block until all queued work of the context is done. */
+int annie_hip_sync(annie_hip_t* ctx);
+
+/* Diagnostics for kernel-level parity tests (no counterpart in the reference):
+ * copy register `reg` (K planes of n residues, plane-major) to host memory. */
+int annie_hip_read_register(annie_hip_t* ctx, size_t reg, uint32_t* out, size_t cap_words);
+int annie_hip_write_register(annie_hip_t* ctx, size_t reg, const uint32_t* in, size_t words);
+
+/* Diagnostics (no counterpart in the reference): run one squaring of r0 up to
+ * an intermediate stage (1: convolutions in registers 5/6, 2: T_R in register
+ * 4 and T_Q digits, 3: m' convolution in register 5, 4: m' + 2 in register 4),
+ * without updating r0; and read the T_Q digit planes. */
+int annie_hip_debug_square_stage(annie_hip_t* ctx, int32_t a, int stage);
+int annie_hip_read_tq(annie_hip_t* ctx, uint32_t* out, size_t cap_words);
+
+/* Releases the context (the reference deletes the transform, src/annie.h:406). */
+void annie_hip_destroy(annie_hip_t* ctx);
+
+const char* annie_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANNIE_HIP_H */
