@@ -74,6 +74,13 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_write_register.argtypes = [h, sz, ctypes.POINTER(ctypes.c_uint32), sz]
     lib.annie_hip_debug_square_stage.argtypes = [h, ctypes.c_int32, ctypes.c_int]
     lib.annie_hip_read_tq.argtypes = [h, ctypes.POINTER(ctypes.c_uint32), sz]
+    lib.annie_hip_time_square_mul_bits.argtypes = [h, ctypes.POINTER(ctypes.c_uint8), sz, ctypes.c_int32,
+                                                    ctypes.POINTER(ctypes.c_float)]
+    lib.annie_hip_profile_begin.argtypes = [h]
+    lib.annie_hip_profile_end.argtypes = [h, ctypes.c_char_p, sz]
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    lib.annie_hip_plan_constants.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, sz]
+    lib.annie_hip_twiddles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, u32p, sz]
     lib.annie_hip_destroy.argtypes = [h]
     lib.annie_hip_destroy.restype = None
     lib.annie_hip_last_error.restype = ctypes.c_char_p
@@ -94,6 +101,28 @@ def select_plan(N, a):
     if rc != ANNIE_OK:
         raise RuntimeError(_err(lib))
     return K.value, pck.value, LN.value, lb.value
+
+
+def plan_constants(LN, K, pck, lbase):
+    """Per-prime constants of a plan (see annie_hip_plan_constants)."""
+    lib = load_library()
+    out = (ctypes.c_uint32 * 50)()
+    if lib.annie_hip_plan_constants(LN, K, pck, lbase, out, 50) != ANNIE_OK:
+        raise RuntimeError(_err(lib))
+    v = list(out)
+    return {"p": v[0:5], "q": v[5:10], "sq_scale": v[10:15], "c32": v[15:20], "c64": v[20:25],
+            "ginv": [v[25 + 5 * i:30 + 5 * i] for i in range(5)]}
+
+
+def twiddles(LN, k, table):
+    """(w, wq) lists of a heap-indexed twiddle table (Montgomery form)."""
+    lib = load_library()
+    n = 1 << LN
+    w = (ctypes.c_uint32 * n)()
+    wq = (ctypes.c_uint32 * n)()
+    if lib.annie_hip_twiddles(LN, k, table, w, wq, n) != ANNIE_OK:
+        raise RuntimeError(_err(lib))
+    return list(w), list(wq)
 
 
 class Transform:
@@ -144,6 +173,26 @@ class Transform:
         """r0 = (bit ? a : 1) * r0^2 for each bit of `bits` (bytes or list of 0/1)."""
         buf = (ctypes.c_uint8 * len(bits))(*bits)
         self._check(self._lib.annie_hip_square_mul_bits(self._h, buf, len(bits), int(a)))
+
+    def time_square_mul_bits(self, bits, a):
+        """As square_mul_bits; returns the GPU time of the sequence in ms (HIP events)."""
+        buf = (ctypes.c_uint8 * len(bits))(*bits)
+        ms = ctypes.c_float()
+        self._check(self._lib.annie_hip_time_square_mul_bits(self._h, buf, len(bits), int(a), ctypes.byref(ms)))
+        return ms.value
+
+    def profile_begin(self):
+        self._check(self._lib.annie_hip_profile_begin(self._h))
+
+    def profile_end(self):
+        """{kernel name: (launches, total ms)} since profile_begin()."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(self._lib.annie_hip_profile_end(self._h, buf, 1 << 16))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, tot = line.split()
+            out[name] = (int(cnt), float(tot))
+        return out
 
     def mul(self, dst, src):
         self._check(self._lib.annie_hip_mul(self._h, dst, src))
@@ -211,3 +260,64 @@ class Transform:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class CheckOpts(ctypes.Structure):
+    _fields_ = [("is_euler", ctypes.c_int), ("a_ref", ctypes.c_int32), ("device", ctypes.c_int),
+                ("gmp_threshold_digits", ctypes.c_int), ("min_ln", ctypes.c_int), ("workdir", ctypes.c_char_p),
+                ("checkpoint_seconds", ctypes.c_double), ("quit_after", ctypes.c_long),
+                ("corrupt_at", ctypes.c_long), ("verbose", ctypes.c_int)]
+
+
+class CheckResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("is_prp", ctypes.c_int), ("res64", ctypes.c_uint64),
+                ("gl_ok", ctypes.c_int), ("a", ctypes.c_int32), ("resumed", ctypes.c_int),
+                ("iterations", ctypes.c_long), ("seconds", ctypes.c_double), ("message", ctypes.c_char * 1024)]
+
+
+STATUS = {0: "success", 1: "failed", 2: "aborted", 3: "factor", 4: "decided", -1: "error"}
+_host = None
+
+
+def load_host_library(path=HOST_LIB_PATH):
+    """Load libannie_host.so (the test protocol host over the engine)."""
+    global _host
+    if _host is None:
+        load_library()
+        if not os.path.exists(path):
+            raise OSError(f"{path} is missing: build it with `python -m annie_amd.build`")
+        lib = ctypes.CDLL(path)
+        lib.annie_check.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.POINTER(CheckOpts),
+                                    ctypes.POINTER(CheckResult)]
+        lib.annie_checkpoint_name.argtypes = [ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t, ctypes.c_char_p,
+                                              ctypes.c_size_t]
+        lib.annie_crc32.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t]
+        lib.annie_crc32.restype = ctypes.c_uint32
+        _host = lib
+    return _host
+
+
+def check(N, is_euler=False, a=0, device=0, gmp_threshold_digits=10000, min_ln=10, workdir=None,
+          checkpoint_seconds=600.0, quit_after=-1, corrupt_at=-1, verbose=False):
+    """Run the probable-primality protocol of annie::check (src/annie.h:217-430) on N.
+
+    Returns a dict: status, is_prp, res64, gl_ok, a, resumed, iterations, seconds, message.
+    """
+    lib = load_host_library()
+    arr, n = int_to_limbs(N)
+    o = CheckOpts(1 if is_euler else 0, a, device, gmp_threshold_digits, min_ln,
+                  workdir.encode() if workdir else None, checkpoint_seconds, quit_after, corrupt_at,
+                  1 if verbose else 0)
+    r = CheckResult()
+    lib.annie_check(arr, n, ctypes.byref(o), ctypes.byref(r))
+    return {"status": STATUS.get(r.status, r.status), "is_prp": bool(r.is_prp), "res64": r.res64,
+            "gl_ok": bool(r.gl_ok), "a": r.a, "resumed": bool(r.resumed), "iterations": r.iterations,
+            "seconds": r.seconds, "message": r.message.decode(errors="replace")}
+
+
+def request_quit():
+    load_host_library().annie_request_quit()
+
+
+def clear_quit():
+    load_host_library().annie_clear_quit()

@@ -73,6 +73,11 @@ struct Ctx {
   PrimeConsts pc;
   NttArgs nbase{};
   CarryArgs cbase{};
+  // Per-kernel event timing (diagnostics / benchmark roofline).
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;        // pool, two per record
+  std::vector<int> rec_kind;         // kernel kind of each record
+  std::vector<std::string> kinds;    // kind names
 
   uint32_t* reg(int r) const { return regs + size_t(r) * size_t(pl.K) * n; }
   uint32_t* tq() const { return raw; }
@@ -99,6 +104,40 @@ static NttJob job(const uint32_t* src, uint32_t* dst, const TwDev* f, const TwDe
   j.inv = i;
   j.opnd = op;
   return j;
+}
+
+// Records an event pair around the next launch when profiling is on.
+struct ProfScope {
+  Ctx& c;
+  size_t rec = size_t(-1);
+  ProfScope(Ctx& cc, const std::string& name) : c(cc) {
+    if (!c.profiling) return;
+    int kind = -1;
+    for (size_t i = 0; i < c.kinds.size(); ++i)
+      if (c.kinds[i] == name) kind = int(i);
+    if (kind < 0) {
+      kind = int(c.kinds.size());
+      c.kinds.push_back(name);
+    }
+    rec = c.rec_kind.size();
+    while (c.ev.size() < 2 * (rec + 1)) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        rec = size_t(-1);
+        return;
+      }
+      c.ev.push_back(e);
+    }
+    c.rec_kind.push_back(kind);
+    (void)hipEventRecord(c.ev[2 * rec], c.stream);
+  }
+  ~ProfScope() {
+    if (rec != size_t(-1)) (void)hipEventRecord(c.ev[2 * rec + 1], c.stream);
+  }
+};
+
+static const char* row_mode_name(int mode) {
+  return mode == ROW_FWD ? "fwd" : mode == ROW_SQR ? "sqr" : mode == ROW_MULC ? "mulc" : "mulv";
 }
 
 template <int M>
@@ -129,6 +168,7 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   A.cols = int(lines);
   const dim3 grid(rows / lines, c.pl.K, nj), block(lines * Tl);
   const size_t shm = size_t(lines) * (S + (S >> 5)) * sizeof(uint32_t);
+  ProfScope ps(c, "k_row<" + std::to_string(c.m2) + "," + row_mode_name(mode) + ">x" + std::to_string(nj));
   switch (c.m2) {
     case 5: launch_row_m<5>(mode, grid, block, shm, c.stream, A); break;
     case 6: launch_row_m<6>(mode, grid, block, shm, c.stream, A); break;
@@ -158,6 +198,7 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   A.cols = int(C);
   const dim3 grid(n2 / C, c.pl.K, nj), block(C * Tl);
   const size_t shm = size_t(C) * S * sizeof(uint32_t);
+  ProfScope ps(c, std::string(inverse ? "k_col_inv<" : "k_col_fwd<") + std::to_string(c.m1) + ">x" + std::to_string(nj));
   switch (c.m1) {
     case 7: launch_col_m<7>(inverse, grid, block, shm, c.stream, A); break;
     case 8: launch_col_m<8>(inverse, grid, block, shm, c.stream, A); break;
@@ -206,6 +247,7 @@ static int launch_carry(Ctx& c, const CarryJob* jobs, int nj) {
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
   const size_t nb = c.n / CARRY_L;
   const dim3 grid(unsigned((nb + CARRY_WG - 1) / CARRY_WG), nj), block(CARRY_WG);
+  ProfScope ps(c, "k_carry<" + std::to_string(c.pl.K) + ">x" + std::to_string(nj));
   switch (c.pl.K) {
     case 2: hipLaunchKernelGGL(k_carry<2>, grid, block, 0, c.stream, A); break;
     case 3: hipLaunchKernelGGL(k_carry<3>, grid, block, 0, c.stream, A); break;
@@ -403,6 +445,7 @@ static void destroy(annie_hip_t* h) {
   if (c.regs) (void)hipFree(c.regs);
   if (c.raw) (void)hipFree(c.raw);
   if (c.tw) (void)hipFree(c.tw);
+  for (hipEvent_t e : c.ev) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   mpz_clears(c.N, c.factor, nullptr);
   delete h;
@@ -543,6 +586,37 @@ int annie_hip_select_plan(const uint64_t* Nl, size_t nl, int32_t a, int* K, int*
   return ANNIE_OK;
 }
 
+int annie_hip_plan_constants(int LN, int K, int pck, int lbase, uint32_t* out, size_t cap) {
+  if (!out || cap < 5 * KMAX + KMAX * KMAX || LN < 1 || LN > 24) return fail(ANNIE_ERR_ARG, "bad argument");
+  Plan pl;
+  pl.K = K;
+  pl.pck = pck;
+  pl.LN = LN;
+  pl.lbase = lbase;
+  const PrimeConsts c = make_consts(pl);
+  for (int k = 0; k < KMAX; ++k) {
+    out[k] = c.p[k];
+    out[KMAX + k] = c.q[k];
+    out[2 * KMAX + k] = c.sq_scale[k];
+    out[3 * KMAX + k] = c.c32[k];
+    out[4 * KMAX + k] = c.c64[k];
+    for (int j = 0; j < KMAX; ++j) out[5 * KMAX + k * KMAX + j] = (j == k) ? 0u : c.ginv[k][j];
+  }
+  return ANNIE_OK;
+}
+
+int annie_hip_twiddles(int LN, int k, int table, uint32_t* w, uint32_t* wq, size_t cap) {
+  if (LN < 1 || LN > 23 || k < 0 || k >= KMAX || table < 0 || table > 3 || !w || !wq || cap < (size_t(1) << LN))
+    return fail(ANNIE_ERR_ARG, "bad argument");
+  const PrimeTables t = make_tables(LN, k);
+  const std::vector<Tw>& v = table == 0 ? t.fwdR : table == 1 ? t.invR : table == 2 ? t.fwdQ : t.invQ;
+  for (size_t i = 0; i < v.size(); ++i) {
+    w[i] = v[i].w;
+    wq[i] = v[i].wq;
+  }
+  return ANNIE_OK;
+}
+
 int annie_hip_create(const uint64_t* N, size_t nlimbs, int32_t a, int device, int min_ln, annie_hip_t** out) {
   annie_hip_t* h = nullptr;
   const int rc = annie::create(N, nlimbs, a, device, min_ln, &h);
@@ -674,6 +748,62 @@ int annie_hip_write_register(annie_hip_t* h, size_t reg, const uint32_t* in, siz
   if (reg > 2 || !in || words != size_t(c.pl.K) * c.n) return fail(ANNIE_ERR_ARG, "bad register write");
   HIPCHK(hipMemcpyAsync(c.reg(int(reg)), in, words * 4, hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  return ANNIE_OK;
+}
+
+// Squarings of annie_hip_square_mul_bits timed with HIP events on the
+// context's stream (the elapsed GPU time of the whole sequence, in ms).
+int annie_hip_time_square_mul_bits(annie_hip_t* h, const uint8_t* bits, size_t count, int32_t a, float* ms) {
+  if (!ready(h)) return ANNIE_ERR_ARG;
+  if (a < 1 || (count && !bits) || !ms) return fail(ANNIE_ERR_ARG, "bad argument");
+  Ctx& c = h->c;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c.stream));
+  int rc = ANNIE_OK;
+  for (size_t i = 0; i < count && rc == ANNIE_OK; ++i) rc = annie::square_mul(c, bits[i] ? uint32_t(a) : 1u);
+  HIPCHK(hipEventRecord(e1, c.stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+// Per-kernel timing: begin clears and enables event pairs around every launch;
+// end synchronises and writes one line per kernel kind, "name count total_ms".
+int annie_hip_profile_begin(annie_hip_t* h) {
+  if (!ready(h)) return ANNIE_ERR_ARG;
+  h->c.profiling = true;
+  h->c.rec_kind.clear();
+  return ANNIE_OK;
+}
+
+int annie_hip_profile_end(annie_hip_t* h, char* buf, size_t cap) {
+  if (!ready(h)) return ANNIE_ERR_ARG;
+  Ctx& c = h->c;
+  c.profiling = false;
+  HIPCHK(hipStreamSynchronize(c.stream));
+  std::vector<double> tot(c.kinds.size(), 0.0);
+  std::vector<long> cnt(c.kinds.size(), 0);
+  for (size_t r = 0; r < c.rec_kind.size(); ++r) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev[2 * r], c.ev[2 * r + 1]));
+    tot[c.rec_kind[r]] += ms;
+    cnt[c.rec_kind[r]] += 1;
+  }
+  std::string out;
+  for (size_t k = 0; k < c.kinds.size(); ++k) {
+    if (!cnt[k]) continue;
+    char line[256];
+    snprintf(line, sizeof(line), "%s %ld %.6f\n", c.kinds[k].c_str(), cnt[k], tot[k]);
+    out += line;
+  }
+  c.rec_kind.clear();
+  if (!buf || cap == 0) return fail(ANNIE_ERR_ARG, "null buffer");
+  std::strncpy(buf, out.c_str(), cap - 1);
+  buf[cap - 1] = 0;
   return ANNIE_OK;
 }
 

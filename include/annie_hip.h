@@ -39,6 +39,15 @@ enum { ANNIE_REG0 = 0, ANNIE_REG1 = 1, ANNIE_REG2 = 2 };
  * (src/transformGPU.h:822-850).  No GPU needed. */
 int annie_hip_select_plan(const uint64_t* N, size_t nlimbs, int32_t a, int* K, int* pck, int* LN, int* lbase);
 
+/* Host-side tables (no GPU needed), for tests.  Constants of a plan: out[0..4]
+ * primes, [5..9] p^-1 mod 2^32, [10..14] n^-1 2^64 mod p, [15..19] 2^64 mod p,
+ * [20..24] 2^96 mod p, [25 + 5i + j] p_i^-1 mod p_j in Montgomery form (the
+ * reference's INVPi_Pj, src/transformGPU.h:699-705).  Twiddles: table 0..3 =
+ * cyclic forward, cyclic inverse, negacyclic forward, negacyclic inverse
+ * (Montgomery form w and w * p^-1 mod 2^32), heap-indexed, n entries. */
+int annie_hip_plan_constants(int LN, int K, int pck, int lbase, uint32_t* out, size_t cap);
+int annie_hip_twiddles(int LN, int k, int table, uint32_t* w, uint32_t* wq, size_t cap);
+
 /* Replaces create_transformGPU(N, a, device) (src/transformGPU.h:822-885) and the
  * transformGPU constructor (:644-753).  min_ln is the smallest accepted log2
  * transform length (the reference accepts 10..23, or 4..23 in its QUICK_TEST
@@ -61,6 +70,17 @@ int annie_hip_square_mul(annie_hip_t* ctx, int32_t a);
 /* The reference's hot loop body for `count` exponent bits (src/annie.h:319-341 without
  * the Gerbicz-Li products): for i < count, r0 = (bits[i] ? a : 1) * r0^2. */
 int annie_hip_square_mul_bits(annie_hip_t* ctx, const uint8_t* bits, size_t count, int32_t a);
+
+/* As annie_hip_square_mul_bits, timed with HIP events on the context's stream:
+ * *ms = GPU time of the whole sequence (no counterpart in the reference; its
+ * bench mode times square_mul with a wall clock, src/annie.h:515-547). */
+int annie_hip_time_square_mul_bits(annie_hip_t* ctx, const uint8_t* bits, size_t count, int32_t a, float* ms);
+
+/* Per-kernel timing with HIP events (the reference's dormant OpenCL profiling
+ * queue, src/ocl.h:330-374, 579-597): begin enables it, end synchronises and
+ * writes one text line per kernel kind: "name count total_ms". */
+int annie_hip_profile_begin(annie_hip_t* ctx);
+int annie_hip_profile_end(annie_hip_t* ctx, char* buf, size_t cap);
 
 /* transform::mul -- r_dst *= r_src (src/transform.h:29; src/transformGPU.h:598-601). */
 int annie_hip_mul(annie_hip_t* ctx, size_t dst, size_t src);

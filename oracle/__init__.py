@@ -87,6 +87,7 @@ def lib():
         L.on_debug_square_stage.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, u32p, u32p]
         L.on_ntt.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p]
         L.on_crt.argtypes = [ctypes.c_int, u32p, u64p]
+        L.on_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p]
         _lib = L
     return _lib
 
@@ -329,6 +330,13 @@ def ntt_plane(LN, k, nega, inverse, values):
     buf = (ctypes.c_uint32 * len(values))(*values)
     lib().on_ntt(LN, k, 1 if nega else 0, 1 if inverse else 0, buf)
     return list(buf)
+
+
+def root_table(LN, k, t):
+    """Oracle root table t (0 cyc fwd, 1 cyc inv, 2 nega fwd, 3 nega inv), plain residues."""
+    out = (ctypes.c_uint32 * (1 << LN))()
+    lib().on_tables(LN, k, t, out)
+    return list(out)
 
 
 def crt(K, residues):

@@ -571,6 +571,17 @@ void on_ntt(int LN, int k, int nega, int inverse, uint32_t* x) {
   std::memcpy(x, v.data(), o.n * 4);
 }
 
+// Root table `t` (0 cyclic fwd, 1 cyclic inv, 2 negacyclic fwd, 3 negacyclic
+// inv) of prime k, plain residues, heap-indexed.
+void on_tables(int LN, int k, int t, uint32_t* out) {
+  Oracle o;
+  o.K = k + 1;
+  o.LN = LN;
+  o.n = size_t(1) << LN;
+  o.tables();
+  std::memcpy(out, o.tw[t][k].data(), o.n * 4);
+}
+
 // Signed CRT value of K residues as 3 64-bit limbs (two's complement, 192 bits).
 void on_crt(int K, const uint32_t* r, uint64_t* out) {
   mpz_t M, Mh, v, t, inv, pk, Mk;

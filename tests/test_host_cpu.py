@@ -1,0 +1,117 @@
+"""CPU tests of the native libraries: the C ABI surface, the protocol host on
+its GMP path (N below 10,000 digits, as the reference), checkpoint files and
+failure handling, and the command-line front end."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import annie_amd
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def _declared(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(annie_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,path", [("annie_hip.h", annie_amd.HIP_LIB_PATH),
+                                         ("annie_host.h", annie_amd.HOST_LIB_PATH)])
+def test_library_exports_every_declared_symbol(header, path):
+    lib = ctypes.CDLL(path)
+    names = _declared(header)
+    assert names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_plan_entry_needs_no_gpu():
+    assert annie_amd.select_plan((1 << 132899999) | 1, 3) == (5, 4, 21, 16)
+
+
+def _fib(n):
+    a, b = 0, 1
+    for _ in range(n):
+        a, b = b, a + b
+    return a
+
+
+@pytest.mark.parametrize("i", [359, 802, 1601, 2971])
+def test_host_protocol_gmp_path(i):
+    N = _fib(i)
+    r = annie_amd.check(N)
+    o = oracle.check(N)
+    assert r["status"] == "success"
+    assert r["is_prp"] == o["is_prp"]
+    assert r["res64"] == o["res64"]
+    assert r["gl_ok"]
+
+
+def test_host_protocol_small_and_even():
+    assert annie_amd.check(1000003)["status"] == "decided"
+    assert annie_amd.check(1000003)["is_prp"]
+    assert annie_amd.check(2 ** 70)["message"].endswith("is even.")
+
+
+def test_host_protocol_euler_proth():
+    # 3*2^189+1 is prime (Proth); Euler test with the reference's base choice.
+    N = 3 * 2 ** 189 + 1
+    r = annie_amd.check(N, is_euler=True)
+    assert r["is_prp"] and r["gl_ok"]
+    assert r["a"] == oracle.base(N, True)
+
+
+def test_host_protocol_detects_corruption():
+    """A register corrupted mid-loop must make the Gerbicz-Li check fail (src/annie.h:343-404)."""
+    N = _fib(2971)
+    r = annie_amd.check(N, corrupt_at=1000)
+    assert r["status"] == "failed"
+    assert not r["gl_ok"]
+
+
+def test_host_protocol_abort(tmp_path):
+    r = annie_amd.check(_fib(2971), workdir=str(tmp_path), quit_after=100)
+    assert r["status"] == "aborted" and r["iterations"] == 100
+
+
+def test_crc32_matches_reference_algorithm():
+    """src/file.h:56-90 (reflected 0xedb88320); zlib's crc32 is the same function."""
+    import zlib
+    lib = annie_amd.load_host_library()
+    for data in (b"", b"a", b"annie checkpoint", bytes(range(256)) * 3):
+        assert lib.annie_crc32(0, data, len(data)) == zlib.crc32(data)
+    # running value: crc(a + b) == crc(crc(a), b)
+    a, b = b"first part", b"second part"
+    assert lib.annie_crc32(lib.annie_crc32(0, a, len(a)), b, len(b)) == zlib.crc32(a + b)
+
+
+def test_checkpoint_name_closed_form():
+    """a_<16 hex>.chk with hash = floor(mantissa * 2^52) * digits + limb0 (src/annie.h:288-291)."""
+    import math
+    N = _fib(9311)
+    lib = annie_amd.load_host_library()
+    arr, n = annie_amd.int_to_limbs(N)
+    buf = ctypes.create_string_buffer(64)
+    assert lib.annie_checkpoint_name(arr, n, buf, 64) == 0
+    m, e = math.frexp(float(N >> (N.bit_length() - 60)))  # mantissa in [0.5, 1)
+    h = (int(m * 2 ** 52) * len(str(N)) + (N & (2 ** 64 - 1))) % 2 ** 64
+    assert buf.value.decode() == f"a_{h:016X}.chk"
+
+
+def test_cli_expression_front_end():
+    exe = os.path.join(annie_amd.LIBDIR, "annie")
+    out = subprocess.run([exe, "-q", "fibonacci(359)", "-w", "/tmp"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "is a Fermat probable prime" in out.stdout
+    out = subprocess.run([exe, "-q", "(2^127-1)*3+2*0", "-w", "/tmp"], capture_output=True, text=True, timeout=120)
+    assert "composite" in out.stdout
+    out = subprocess.run([exe, "-q", "10!+1"], capture_output=True, text=True, timeout=60)
+    assert "3628801 is" in out.stdout
+    out = subprocess.run([exe, "-q", "unknown(3)"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "unknown function" in out.stderr
