@@ -36,7 +36,11 @@ constexpr uint32_t PRIMES[KMAX] = {
 // against the factorisation of p - 1).
 constexpr uint32_t PRIMROOTS[KMAX] = {3, 7, 3, 3, 3};
 
+#if defined(__HIP_DEVICE_COMPILE__)
+ANNIE_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
+#else
 ANNIE_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return uint32_t((uint64_t(a) * b) >> 32); }
+#endif
 
 // p^-1 mod 2^32 by Newton iteration (each step doubles the correct low bits).
 ANNIE_HD uint32_t inv32(uint32_t p) {
@@ -45,19 +49,18 @@ ANNIE_HD uint32_t inv32(uint32_t p) {
   return q;
 }
 
-// a + b mod p for a, b in [0, p), p < 2^32 (the sum may carry out of 32 bits).
-ANNIE_HD uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) {
-  const uint32_t s = a + b;
-  const uint32_t t = s - p;
-  // a + b >= p  <=>  the 32-bit sum carried, or s >= p.
-  return (s < a || s >= p) ? t : s;
-}
+// The corrections below are written with the carry / borrow builtins so that
+// the device code is a subtract-with-borrow followed by a select of p or 0.
 
 // a - b mod p for a, b in [0, p).
 ANNIE_HD uint32_t submod(uint32_t a, uint32_t b, uint32_t p) {
-  const uint32_t d = a - b;
-  return (a < b) ? d + p : d;
+  uint32_t d;
+  const bool borrow = __builtin_sub_overflow(a, b, &d);
+  return d + (borrow ? p : 0u);
 }
+
+// a + b mod p for a, b in [0, p), p < 2^32: a - (p - b) mod p.
+ANNIE_HD uint32_t addmod(uint32_t a, uint32_t b, uint32_t p) { return submod(a, p - b, p); }
 
 // Montgomery product by a constant: a * w * 2^-32 mod p, with wq = w * p^-1 mod 2^32.
 // Valid for any a < 2^32 and w < p.
@@ -65,8 +68,7 @@ ANNIE_HD uint32_t mont_c(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {
   const uint32_t h = mulhi32(a, w);
   const uint32_t m = a * wq;
   const uint32_t g = mulhi32(m, p);
-  const uint32_t r = h - g;
-  return (h < g) ? r + p : r;
+  return submod(h, g, p);
 }
 
 // General Montgomery product a * b * 2^-32 mod p (q = p^-1 mod 2^32).
