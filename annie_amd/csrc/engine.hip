@@ -281,7 +281,7 @@ static int reduce(Ctx& c, uint32_t* dst, bool raw_out) {
   if ((rc = launch_carry(c, &cj, 1))) return rc;
   j = job(c.reg(RTR), c.reg(RWQ), c.fwdQ(), c.invQ(), c.reg(RNQ));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
-  cj = cjob(c.reg(RWQ), dst, 1u, 1, 2, raw_out ? 1 : 0, c.tq());
+  cj = cjob(c.reg(RWQ), dst, 1u, 1, 0, raw_out ? 1 : 0, c.tq());
   return launch_carry(c, &cj, 1);
 }
 

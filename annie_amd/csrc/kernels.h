@@ -47,7 +47,7 @@ struct CarryJob {
   const uint32_t* sub;  // DIGIT_LIMBS raw digit planes subtracted from each coefficient, or null
   uint32_t a;           // multiplier of every coefficient
   int nega;             // 1: value mod B^n + 1 (wrap negated), 0: mod B^n - 1
-  int canon;            // 0: none, 1: reduce into [0, R) then add 2, 2: reduce into [0, Q)
+  int canon;            // 1: reduce m' into [0, R) then add 2; 0 otherwise (mod Q needs no fix-up)
   int raw_out;          // 1: write raw digits instead of residues
 };
 

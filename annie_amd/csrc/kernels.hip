@@ -327,9 +327,12 @@ __global__ void k_scale(uint32_t* x, size_t n, int K, NttArgs A, uint32_t s0, ui
 // the wrapped carry of the last block (negated for the negacyclic product).
 //
 // Canonical range.  mod Q the result r' < 4N < B^n/4 is the only value of its
-// class in (-B^n/2, B^n/2), so it comes out exact; mod R, m' is recovered
-// from the sign of the number (that of its most significant nonzero digit,
-// the scan of src/transformCPU.h:1261-1281): a negative value gets R added.
+// class in (-B^n/2, B^n/2), so it comes out exact with no fix-up (its top
+// digits are zero: a sign scan there would walk them all); mod R, m' (uniform
+// in [0, R)) is recovered from the sign of the number, that of its most
+// significant nonzero digit (the scan of src/transformCPU.h:1261-1281, which
+// stops at the top digit except with probability ~2^-cbits): a negative value
+// gets R added.
 // ---------------------------------------------------------------------------
 
 template <int NL>
@@ -620,12 +623,13 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   }
   carry_block_finish<NL>(dig, fin, A.cbits);
 
-  if (J.canon != 0 && (b == 0 || b == nb - 1)) {
-    // Both ends of the number take the same decision: add R = B^n - 1 (canon 1)
-    // or Q = B^n + 1 (canon 2) to a negative value, then add 2 (canon 1).
+  if (J.canon == 1 && (b == 0 || b == nb - 1)) {
+    // m' into [0, R): both ends of the number take the same decision, adding
+    // R = B^n - 1 to a negative value; then m' + 2.  (mod Q needs nothing: the
+    // result is already the unique member of its class in (-B^n/2, B^n/2).)
     const int sg = number_sign<K, NL>(J, A, n);
     if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits, 1);
-    if (b == 0) digit_add(dig[0], ((sg < 0) ? ((J.canon == 1) ? -1 : 1) : 0) + ((J.canon == 1) ? 2 : 0));
+    if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
   }
 
   const size_t i0 = b * CARRY_L;

@@ -73,6 +73,10 @@ def alg_bytes(kind, U):
     return jobs * 2 * U
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline_worker(skip, seconds):
     """Runs in a child process pinned to one core: the reference transformCPU."""
     try:
@@ -127,6 +131,7 @@ def main():
     skip = 0
     while True:
         N = workload_N(skip=skip)
+        progress(f"candidate {skip}: {N.bit_length()}-bit N; creating the context (N^-1 mod R takes about a minute)")
         t0 = time.time()
         cpu = None
         if rank == 0 and not args.no_cpu_baseline:
@@ -135,6 +140,7 @@ def main():
                                    stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
         T = annie_amd.Transform(N, BASE)
         init_s = time.time() - t0
+        progress(f"context ready in {init_s:.1f} s (init_failed={T.init_failed()})")
         if not T.init_failed():
             break
         if cpu is not None:
@@ -147,13 +153,16 @@ def main():
     total = args.warmup + args.steps + args.profile_steps
     bits = exponent_bits(N, total)
     T.set1()
+    progress(f"warmup: {args.warmup} squarings")
     T.square_mul_bits(bits[:args.warmup], BASE)
     T.sync()
+    progress(f"timed region: {args.steps} squarings")
     w0 = time.time()
     ms = T.time_square_mul_bits(bits[args.warmup:args.warmup + args.steps], BASE)
     wall = time.time() - w0
     ms_per_sq = ms / args.steps
 
+    progress(f"{ms_per_sq:.4f} ms/squaring; profile window: {args.profile_steps} squarings")
     T.profile_begin()
     T.square_mul_bits(bits[args.warmup + args.steps:], BASE)
     prof = T.profile_end()
@@ -166,6 +175,10 @@ def main():
 
     cpu_res = None
     if cpu is not None:
+        progress("waiting for the CPU baseline (reference transformCPU on one core)")
+        while cpu.poll() is None:
+            time.sleep(10)
+            progress("CPU baseline still running")
         out, err = cpu.communicate(timeout=900)
         try:
             cpu_res = json.loads(out.strip().splitlines()[-1])
