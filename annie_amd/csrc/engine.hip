@@ -67,7 +67,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   uint32_t* regs = nullptr;   // NREG * K * n words
   uint32_t* raw = nullptr;    // 2 * DIGIT_LIMBS * n words: T_Q digits, get() digits
-  TwDev* tw = nullptr;        // 4 tables * K * n
+  TwDev* tw = nullptr;        // 4 tables * K * n roots (Montgomery form)
   mpz_t N, factor;
   bool init_failed = false;
   PrimeConsts pc;
@@ -192,7 +192,9 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   A.m1 = c.m1;
   const uint32_t S = 1u << c.m1, Tl = S / NTT_R;
   const uint32_t n2 = uint32_t(c.n >> c.m1);
+  // Up to 16 columns of Tl threads each per block (C * 2^m1 * 4 bytes of LDS).
   uint32_t C = 1024u / Tl;
+  if (C < 1) C = 1;
   if (C > 16) C = 16;
   if (C > n2) C = n2;
   A.cols = int(C);
@@ -519,10 +521,10 @@ static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_
     for (size_t k = 0; k < K; ++k) {
       const PrimeTables pt = make_tables(c.pl.LN, int(k));
       for (size_t i = 0; i < n; ++i) {
-        host[(0 * K + k) * n + i] = TwDev{pt.fwdR[i].w, pt.fwdR[i].wq};
-        host[(1 * K + k) * n + i] = TwDev{pt.invR[i].w, pt.invR[i].wq};
-        host[(2 * K + k) * n + i] = TwDev{pt.fwdQ[i].w, pt.fwdQ[i].wq};
-        host[(3 * K + k) * n + i] = TwDev{pt.invQ[i].w, pt.invQ[i].wq};
+        host[(0 * K + k) * n + i] = pt.fwdR[i].w;
+        host[(1 * K + k) * n + i] = pt.invR[i].w;
+        host[(2 * K + k) * n + i] = pt.fwdQ[i].w;
+        host[(3 * K + k) * n + i] = pt.invQ[i].w;
       }
     }
     HIPCHK(hipMemcpyAsync(c.tw, host.data(), host.size() * sizeof(TwDev), hipMemcpyHostToDevice, c.stream));

@@ -7,11 +7,11 @@
 
 namespace annie {
 
-struct TwDev {
-  uint32_t w, wq;
-};
+// A twiddle in HBM: the root in Montgomery form (w * p^-1 mod 2^32 is formed
+// on the fly with one multiply, which halves the table traffic).
+typedef uint32_t TwDev;
 
-// Elements per thread in the NTT passes (log2 = levels per register round).
+// Elements per thread in the NTT passes (four levels per register round).
 constexpr int NTT_R = 16;
 constexpr int NTT_LOGR = 4;
 // Consecutive coefficients per thread in the carry kernel.

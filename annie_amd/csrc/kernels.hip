@@ -17,43 +17,48 @@
 // Lengths above 2^13 use two passes over HBM per direction: a column pass runs
 // the first m1 levels on lines of stride n2 = n / 2^m1, a row pass runs the
 // remaining levels on contiguous rows of n2 elements, and fuses the pointwise
-// product and the inverse row levels.  Each thread holds 16 elements in
-// registers and runs four levels per LDS round trip.
+// product and the inverse row levels.  Each thread holds 32 elements in
+// registers (two or more independent groups) and runs four levels per LDS
+// round trip.
 #include <type_traits>
 
 #include "kernels.h"
 
 namespace annie {
 
-__device__ __forceinline__ void fwd_bfly(uint32_t& a, uint32_t& b, const TwDev w, const uint32_t p) {
+struct Tw2 {
+  uint32_t w, wq;
+};
+
+__device__ __forceinline__ void fwd_bfly(uint32_t& a, uint32_t& b, const Tw2 w, const uint32_t p) {
   const uint32_t t = mont_c(b, w.w, w.wq, p);
   b = submod(a, t, p);
   a = addmod(a, t, p);
 }
 
-__device__ __forceinline__ void inv_bfly(uint32_t& a, uint32_t& b, const TwDev w, const uint32_t p) {
+__device__ __forceinline__ void inv_bfly(uint32_t& a, uint32_t& b, const Tw2 w, const uint32_t p) {
   const uint32_t s = addmod(a, b, p);
   const uint32_t d = submod(a, b, p);
   a = s;
   b = mont_c(d, w.w, w.wq, p);
 }
 
-__device__ __forceinline__ TwDev ldtw(const TwDev* t, uint32_t i) {
-  const uint2 v = *reinterpret_cast<const uint2*>(t + i);
-  return TwDev{v.x, v.y};
+__device__ __forceinline__ Tw2 ldtw(const TwDev* t, uint32_t i, uint32_t q) {
+  const uint32_t w = t[i];
+  return Tw2{w, w * q};
 }
 
 // Levels [d0, d0 + RR) on one group of 2^RR register elements; H0 is the heap
 // index of the group's node at level d0 (children of heap node h are 2h, 2h+1).
 template <int RR>
-__device__ __forceinline__ void group_fwd(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p) {
+__device__ __forceinline__ void group_fwd(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int e = 0; e < RR; ++e) {
     constexpr int N = 1 << RR;
     const int half = N >> (e + 1);
 #pragma unroll
     for (int blk = 0; blk < (1 << e); ++blk) {
-      const TwDev w = ldtw(tw, (H0 << e) + blk);
+      const Tw2 w = ldtw(tw, (H0 << e) + blk, q);
 #pragma unroll
       for (int jj = 0; jj < half; ++jj) {
         const int j = blk * 2 * half + jj;
@@ -64,14 +69,14 @@ __device__ __forceinline__ void group_fwd(uint32_t* x, uint32_t H0, const TwDev*
 }
 
 template <int RR>
-__device__ __forceinline__ void group_inv(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p) {
+__device__ __forceinline__ void group_inv(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int e = RR - 1; e >= 0; --e) {
     constexpr int N = 1 << RR;
     const int half = N >> (e + 1);
 #pragma unroll
     for (int blk = 0; blk < (1 << e); ++blk) {
-      const TwDev w = ldtw(tw, (H0 << e) + blk);
+      const Tw2 w = ldtw(tw, (H0 << e) + blk, q);
 #pragma unroll
       for (int jj = 0; jj < half; ++jj) {
         const int j = blk * 2 * half + jj;
@@ -83,11 +88,13 @@ __device__ __forceinline__ void group_inv(uint32_t* x, uint32_t H0, const TwDev*
 
 // Position inside the line of element jj of sub-group s held by thread slot tg
 // in a round covering levels [d0, d0 + RR) of a line of 2^m elements.  A thread
-// holds 16 >> RR groups of 2^RR elements; group g spans
+// holds NTT_R >> RR groups of 2^RR elements, group g = tg + s * (2^m / NTT_R)
+// (consecutive threads take consecutive groups, so round 0 reads and the
+// last inverse round writes coalesced); group g spans
 // node*2^{m-d0} + off + j*2^{m-d0-RR}.
 template <int RR>
 __device__ __forceinline__ uint32_t group_base(int m, int d0, uint32_t tg, int s, uint32_t& node) {
-  const uint32_t g = tg * (uint32_t(NTT_R) >> RR) + uint32_t(s);
+  const uint32_t g = tg + (uint32_t(s) << (m - NTT_LOGR));
   const int lst = m - d0 - RR;
   node = g >> lst;
   const uint32_t off = g & ((1u << lst) - 1u);
@@ -121,23 +128,23 @@ __device__ __forceinline__ void round_store(const uint32_t (&x)[NTT_R], int m, i
 
 template <int RR>
 __device__ __forceinline__ void round_fwd(uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, uint32_t lineRoot,
-                                          const TwDev* tw, uint32_t p) {
+                                          const TwDev* tw, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int s = 0; s < (NTT_R >> RR); ++s) {
     uint32_t node;
     group_base<RR>(m, d0, tg, s, node);
-    group_fwd<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p);
+    group_fwd<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p, q);
   }
 }
 
 template <int RR>
 __device__ __forceinline__ void round_inv(uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, uint32_t lineRoot,
-                                          const TwDev* tw, uint32_t p) {
+                                          const TwDev* tw, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int s = 0; s < (NTT_R >> RR); ++s) {
     uint32_t node;
     group_base<RR>(m, d0, tg, s, node);
-    group_inv<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p);
+    group_inv<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p, q);
   }
 }
 
@@ -156,15 +163,15 @@ struct Sched {
 // elements in registers.
 template <int M, int I, class GLd, class LLd, class LSt>
 __device__ __forceinline__ void fwd_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
-                                           uint32_t p, GLd& gld, LLd& lld, LSt& lst) {
+                                           uint32_t p, uint32_t q, GLd& gld, LLd& lld, LSt& lst) {
   constexpr int RR = Sched<M>::template rr<I>();
   if constexpr (I == 0) round_load<RR>(x, M, 4 * I, tg, gld);
   else round_load<RR>(x, M, 4 * I, tg, lld);
-  round_fwd<RR>(x, M, 4 * I, tg, lineRoot, tw, p);
+  round_fwd<RR>(x, M, 4 * I, tg, lineRoot, tw, p, q);
   if constexpr (I + 1 < Sched<M>::NR) {
     round_store<RR>(x, M, 4 * I, tg, lst);
     __syncthreads();
-    fwd_rounds<M, I + 1>(x, tg, lineRoot, tw, p, gld, lld, lst);
+    fwd_rounds<M, I + 1>(x, tg, lineRoot, tw, p, q, gld, lld, lst);
   }
 }
 
@@ -172,14 +179,14 @@ __device__ __forceinline__ void fwd_rounds(uint32_t (&x)[NTT_R], uint32_t tg, ui
 // last forward round; round 0 stores through gst.
 template <int M, int I, class GSt, class LLd, class LSt>
 __device__ __forceinline__ void inv_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
-                                           uint32_t p, GSt& gst, LLd& lld, LSt& lst) {
+                                           uint32_t p, uint32_t q, GSt& gst, LLd& lld, LSt& lst) {
   constexpr int RR = Sched<M>::template rr<I>();
   if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
-  round_inv<RR>(x, M, 4 * I, tg, lineRoot, tw, p);
+  round_inv<RR>(x, M, 4 * I, tg, lineRoot, tw, p, q);
   if constexpr (I > 0) {
     round_store<RR>(x, M, 4 * I, tg, lst);
     __syncthreads();
-    inv_rounds<M, I - 1>(x, tg, lineRoot, tw, p, gst, lld, lst);
+    inv_rounds<M, I - 1>(x, tg, lineRoot, tw, p, q, gst, lld, lst);
   } else {
     round_store<RR>(x, M, 0, tg, gst);
   }
@@ -189,7 +196,7 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 
 // Row pass: the levels [m1, LN) of one contiguous row of 2^M elements per
 // line, then (by mode) the pointwise product and the inverse row levels.
-// Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / 16 threads.
+// Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / NTT_R threads.
 template <int M, int MODE>
 __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
@@ -215,7 +222,7 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
 
   uint32_t x[NTT_R];
-  fwd_rounds<M, 0>(x, tg, lineRoot, twf, p, gld, lld, lst);
+  fwd_rounds<M, 0>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst);
   if constexpr (MODE == ROW_FWD) {
     round_store<RL>(x, M, DL, tg, gst);
   } else {
@@ -238,7 +245,7 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
         }
       }
     }
-    inv_rounds<M, NR - 1>(x, tg, lineRoot, twi, p, gst, lld, lst);
+    inv_rounds<M, NR - 1>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst);
   }
 }
 
@@ -267,7 +274,7 @@ __global__ void __launch_bounds__(1024) k_col_fwd(const NttArgs A) {
   auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
 
   uint32_t x[NTT_R];
-  fwd_rounds<M, 0>(x, tg, 1u, twf, p, gld, lld, lst);
+  fwd_rounds<M, 0>(x, tg, 1u, twf, p, A.q[k], gld, lld, lst);
   round_store<RL>(x, M, DL, tg, gst);
 }
 
@@ -294,7 +301,7 @@ __global__ void __launch_bounds__(1024) k_col_inv(const NttArgs A) {
 
   uint32_t x[NTT_R];
   round_load<RL>(x, M, DL, tg, gld);
-  inv_rounds<M, NR - 1>(x, tg, 1u, twi, p, gst, lld, lst);
+  inv_rounds<M, NR - 1>(x, tg, 1u, twi, p, A.q[k], gst, lld, lst);
 }
 
 // Elementwise Montgomery scaling x -> x * s * 2^-32 of K planes (used once at

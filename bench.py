@@ -173,6 +173,11 @@ def main():
     achieved = alg_bytes(dom, U) / (avg_ms * 1e-3) / 1e9
     b_alg_sq = 27 * U  # (16P - 5) U with P = 2 two-pass directions at this length
 
+    # A plain register copy of known size (U bytes read, U written) for
+    # calibrating HBM byte counters in profiling runs.
+    T.copy(1, 0)
+    T.sync()
+
     cpu_res = None
     if cpu is not None:
         progress("waiting for the CPU baseline (reference transformCPU on one core)")
