@@ -28,7 +28,7 @@ template <int M>
 __global__ void k_col_inv(const NttArgs A);
 __global__ void k_scale(uint32_t* x, size_t n, int K, NttArgs A, uint32_t s0, uint32_t s1, uint32_t s2, uint32_t s3,
                         uint32_t s4);
-template <int K>
+template <int K, int QS>
 __global__ void k_carry(const CarryArgs A);
 template <int K>
 __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const CarryArgs A);
@@ -248,13 +248,28 @@ static int launch_carry(Ctx& c, const CarryJob* jobs, int nj) {
   CarryArgs A = c.cbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
   const size_t nb = c.n / CARRY_L;
-  const dim3 grid(unsigned((nb + CARRY_WG - 1) / CARRY_WG), nj), block(CARRY_WG);
+  // CARRY_WG - 1 owned blocks per workgroup (thread 0 only hands over a carry).
+  const dim3 grid(unsigned((nb + CARRY_WG - 2) / (CARRY_WG - 1)), nj), block(CARRY_WG);
+  const int qs = c.pl.cbits() >> 5;
+  if (qs > 2) return fail(ANNIE_ERR_SIZE, "digit width above 95 bits");
   ProfScope ps(c, "k_carry<" + std::to_string(c.pl.K) + ">x" + std::to_string(nj));
-  switch (c.pl.K) {
-    case 2: hipLaunchKernelGGL(k_carry<2>, grid, block, 0, c.stream, A); break;
-    case 3: hipLaunchKernelGGL(k_carry<3>, grid, block, 0, c.stream, A); break;
-    case 4: hipLaunchKernelGGL(k_carry<4>, grid, block, 0, c.stream, A); break;
-    default: hipLaunchKernelGGL(k_carry<5>, grid, block, 0, c.stream, A); break;
+  switch (c.pl.K * 4 + qs) {
+#define ANNIE_CARRY_CASE(K, QS) \
+  case K * 4 + QS: hipLaunchKernelGGL((k_carry<K, QS>), grid, block, 0, c.stream, A); break;
+    ANNIE_CARRY_CASE(2, 0)
+    ANNIE_CARRY_CASE(2, 1)
+    ANNIE_CARRY_CASE(2, 2)
+    ANNIE_CARRY_CASE(3, 0)
+    ANNIE_CARRY_CASE(3, 1)
+    ANNIE_CARRY_CASE(3, 2)
+    ANNIE_CARRY_CASE(4, 0)
+    ANNIE_CARRY_CASE(4, 1)
+    ANNIE_CARRY_CASE(4, 2)
+    ANNIE_CARRY_CASE(5, 0)
+    ANNIE_CARRY_CASE(5, 1)
+    ANNIE_CARRY_CASE(5, 2)
+#undef ANNIE_CARRY_CASE
+    default: return fail(ANNIE_ERR_SIZE, "no carry kernel for this RNS size");
   }
   HIPCHK(hipGetLastError());
   return ANNIE_OK;
@@ -507,6 +522,8 @@ static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_
     c.cbase.c32q[k] = c.pc.c32q[k];
     c.cbase.c64[k] = c.pc.c64[k];
     c.cbase.c64q[k] = c.pc.c64q[k];
+    c.cbase.c32n[k] = c.pc.p[k] - c.pc.c32[k];
+    c.cbase.c32nq[k] = c.cbase.c32n[k] * c.pc.q[k];
     for (int j = 0; j < KMAX; ++j) {
       c.cbase.ginv[k][j] = c.pc.ginv[k][j];
       c.cbase.ginvq[k][j] = c.pc.ginvq[k][j];

@@ -55,6 +55,7 @@ struct CarryArgs {
   CarryJob job[2];
   uint32_t p[KMAX], q[KMAX];
   uint32_t c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX];
+  uint32_t c32n[KMAX], c32nq[KMAX];  // p - c32 (2^64 mod p negated) and its product with p^-1
   uint32_t ginv[KMAX][KMAX], ginvq[KMAX][KMAX];
   int LN;
   int cbits;  // bits per coefficient (pck * lbase)

@@ -249,6 +249,17 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   }
 }
 
+// Column group of a column-pass workgroup.  Workgroups are dispatched to the
+// eight XCDs round-robin (XCD = workgroup id mod 8), and each XCD has its own
+// L2; a group of 16 columns covers half of a 128-byte line, so workgroups x
+// and x + 8 (same XCD, dispatched together) take the two halves of one line:
+// within each run of 16 ids, x -> 2 (x mod 8) + (x div 8 mod 2).
+__device__ __forceinline__ uint32_t col_group() {
+  const uint32_t x = blockIdx.x;
+  if (gridDim.x % 16u != 0u) return x;
+  return (x & ~15u) | ((x & 7u) << 1) | ((x >> 3) & 1u);
+}
+
 // Column passes: the first M levels on columns of stride n2 = n / 2^M; `cols`
 // consecutive columns per block, interleaved in LDS so that global accesses
 // of a wavefront touch consecutive words.  Grid: (n2 / cols, K, jobs).
@@ -263,7 +274,7 @@ __global__ void __launch_bounds__(1024) k_col_fwd(const NttArgs A) {
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   const size_t n2 = n >> M;
-  const size_t base = size_t(k) * n + size_t(blockIdx.x) * C + line;
+  const size_t base = size_t(k) * n + size_t(col_group()) * C + line;
   const TwDev* twf = J.fwd + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
@@ -289,7 +300,7 @@ __global__ void __launch_bounds__(1024) k_col_inv(const NttArgs A) {
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   const size_t n2 = n >> M;
-  const size_t base = size_t(k) * n + size_t(blockIdx.x) * C + line;
+  const size_t base = size_t(k) * n + size_t(col_group()) * C + line;
   const TwDev* twi = J.inv + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
@@ -328,10 +339,12 @@ __global__ void k_scale(uint32_t* x, size_t n, int K, NttArgs A, uint32_t s0, ui
 // CARRY_L - 1 digits stays in the last digit, so every digit ends in
 // [-B/2 - 1, B/2).  The value is preserved exactly, a carry never travels
 // more than one block (the lazy scheme of ocl/kernel.cl:1561-1613), and the
-// whole number lies in (-B^n/2, B^n/2).  The first thread of a workgroup
-// recomputes its predecessor's zero-in carry from that block's residues, so
-// the pass is one launch with no inter-workgroup communication; block 0 takes
-// the wrapped carry of the last block (negated for the negacyclic product).
+// whole number lies in (-B^n/2, B^n/2).  Workgroup w owns the CARRY_WG - 1
+// blocks [w (CARRY_WG - 1), (w + 1)(CARRY_WG - 1)); its thread 0 runs phase 1
+// on the block just before them (the last block, for w = 0) only to hand its
+// zero-in carry to thread 1, so the pass is one launch with no
+// inter-workgroup communication; block 0 takes the wrapped carry of the last
+// block (negated for the negacyclic product).
 //
 // Canonical range.  mod Q the result r' < 4N < B^n/4 is the only value of its
 // class in (-B^n/2, B^n/2), so it comes out exact with no fix-up (its top
@@ -340,103 +353,88 @@ __global__ void k_scale(uint32_t* x, size_t n, int K, NttArgs A, uint32_t s0, ui
 // significant nonzero digit (the scan of src/transformCPU.h:1261-1281, which
 // stops at the top digit except with probability ~2^-cbits): a negative value
 // gets R added.
+//
+// Multi-limb integers are two's complement, little-endian 32-bit limbs.  The
+// digit width cbits = 32 QS + s has its limb shift QS as a template parameter.
 // ---------------------------------------------------------------------------
 
-template <int NL>
-struct Big {
-  uint32_t w[NL];
-};
-
-template <int NL>
-__device__ __forceinline__ void big_set_i32(Big<NL>& a, int32_t v) {
-  a.w[0] = uint32_t(v);
-#pragma unroll
-  for (int i = 1; i < NL; ++i) a.w[i] = uint32_t(v >> 31);
-}
-
-// a = a * m + c (mod 2^{32 NL}), m and c unsigned 32-bit.
-template <int NL>
-__device__ __forceinline__ void big_mad(Big<NL>& a, uint32_t m, uint32_t c) {
+// a = a * m + c (mod 2^{32 L}), m and c unsigned 32-bit.
+template <int L>
+__device__ __forceinline__ void big_mad(uint32_t* a, uint32_t m, uint32_t c) {
   uint64_t carry = c;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const uint64_t t = uint64_t(a.w[i]) * m + carry;
-    a.w[i] = uint32_t(t);
+  for (int i = 0; i < L; ++i) {
+    const uint64_t t = uint64_t(a[i]) * m + carry;
+    a[i] = uint32_t(t);
     carry = t >> 32;
   }
 }
 
-template <int NL>
-__device__ __forceinline__ void big_add(Big<NL>& a, const Big<NL>& b) {
+template <int L>
+__device__ __forceinline__ void big_add(uint32_t* a, const uint32_t* b) {
   uint32_t c = 0;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const uint64_t t = uint64_t(a.w[i]) + b.w[i] + c;
-    a.w[i] = uint32_t(t);
+  for (int i = 0; i < L; ++i) {
+    const uint64_t t = uint64_t(a[i]) + b[i] + c;
+    a[i] = uint32_t(t);
     c = uint32_t(t >> 32);
   }
 }
 
-template <int NL>
-__device__ __forceinline__ void big_sub(Big<NL>& a, const Big<NL>& b) {
+template <int L>
+__device__ __forceinline__ void big_sub(uint32_t* a, const uint32_t* b) {
   uint32_t br = 0;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const uint64_t t = uint64_t(a.w[i]) - b.w[i] - br;
-    a.w[i] = uint32_t(t);
+  for (int i = 0; i < L; ++i) {
+    const uint64_t t = uint64_t(a[i]) - b[i] - br;
+    a[i] = uint32_t(t);
     br = uint32_t(t >> 32) & 1u;
   }
 }
 
-// Sign-extend a DIGIT_LIMBS two's-complement digit to NL limbs.
-template <int NL>
-__device__ __forceinline__ Big<NL> big_from_digit(const uint32_t d[DIGIT_LIMBS]) {
-  Big<NL> r;
+// Sign-extend a DIGIT_LIMBS digit to L limbs.
+template <int L>
+__device__ __forceinline__ void big_from_digit(const uint32_t (&d)[DIGIT_LIMBS], uint32_t* r) {
+  const uint32_t sg = uint32_t(int32_t(d[DIGIT_LIMBS - 1]) >> 31);
 #pragma unroll
-  for (int i = 0; i < NL; ++i) r.w[i] = (i < DIGIT_LIMBS) ? d[i] : uint32_t(int32_t(d[DIGIT_LIMBS - 1]) >> 31);
-  return r;
+  for (int i = 0; i < L; ++i) r[i] = (i < DIGIT_LIMBS) ? d[i] : sg;
 }
 
 // Split f into a balanced digit d = f mod B in [-B/2, B/2) (B = 2^cbits,
-// DIGIT_LIMBS two's-complement limbs) and f <- (f - d) / B.  cbits < 32 *
-// DIGIT_LIMBS; limb shifts are selected without dynamic register indexing.
-template <int NL>
-__device__ __forceinline__ void big_split(Big<NL>& f, int cbits, uint32_t d[DIGIT_LIMBS]) {
-  const int qs = cbits >> 5, s = cbits & 31;
-  const uint32_t sign = uint32_t(int32_t(f.w[NL - 1]) >> 31);
-  const uint32_t mask = (s == 0) ? 0u : ((1u << s) - 1u);
-  const int sl = (cbits - 1) >> 5, sb = (cbits - 1) & 31;
-  const uint32_t wsl = (sl == 0) ? f.w[0] : ((sl == 1) ? f.w[1] : f.w[2]);
-  const uint32_t neg = (wsl >> sb) & 1u;
+// cbits = 32 QS + s, DIGIT_LIMBS limbs) and f <- (f - d) / B.
+template <int L, int QS>
+__device__ __forceinline__ void big_split(uint32_t (&f)[L], uint32_t s, uint32_t (&d)[DIGIT_LIMBS]) {
+  static_assert(QS < L && QS < DIGIT_LIMBS, "digit wider than the accumulator");
+  const uint32_t sign = uint32_t(int32_t(f[L - 1]) >> 31);
+  uint32_t neg;  // bit cbits - 1 of f
+  if constexpr (QS == 0) {
+    neg = (f[0] >> ((s - 1u) & 31u)) & 1u;
+  } else {
+    neg = (s == 0u) ? (f[QS - 1] >> 31) : ((f[QS] >> ((s - 1u) & 31u)) & 1u);
+  }
+  const uint32_t mask = (1u << s) - 1u;
   const uint32_t fill = 0u - neg;
 #pragma unroll
-  for (int i = 0; i < DIGIT_LIMBS; ++i) {
-    const uint32_t wi = (i < NL) ? f.w[i] : sign;
-    d[i] = (i < qs) ? wi : ((i == qs) ? ((wi & mask) | (fill & ~mask)) : fill);
-  }
-  uint32_t g[NL + 1];
-#pragma unroll
-  for (int i = 0; i <= NL; ++i) {
-    const uint32_t a0 = (i < NL) ? f.w[i] : sign;
-    const uint32_t a1 = (i + 1 < NL) ? f.w[i + 1] : sign;
-    const uint32_t a2 = (i + 2 < NL) ? f.w[i + 2] : sign;
-    g[i] = (qs == 0) ? a0 : ((qs == 1) ? a1 : a2);
-  }
+  for (int i = 0; i < DIGIT_LIMBS; ++i) d[i] = (i < QS) ? f[i] : ((i == QS) ? ((f[QS] & mask) | (fill & ~mask)) : fill);
   uint32_t c = neg;
 #pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const uint32_t w = (s == 0) ? g[i] : __builtin_amdgcn_alignbit(g[i + 1], g[i], uint32_t(s));
-    const uint64_t t = uint64_t(w) + c;
-    f.w[i] = uint32_t(t);
+  for (int i = 0; i < L; ++i) {
+    const uint32_t lo = (i + QS < L) ? f[i + QS] : sign;
+    const uint32_t hi = (i + QS + 1 < L) ? f[i + QS + 1] : sign;
+    const uint64_t t = uint64_t(__builtin_amdgcn_alignbit(hi, lo, s)) + c;
+    f[i] = uint32_t(t);
     c = uint32_t(t >> 32);
   }
 }
 
 // Signed value of one coefficient from its K residues (each in [0, p_k)):
 // mixed-radix digits u_{K-1} = r_{K-1}, u_j = (...((r_j - u_{K-1}) / p_{K-1}) - ...) / p_{j+1}
-// mod p_j, with u_0 taken in (-p_0/2, p_0/2], then Horner over the primes.
-template <int K, int NL>
-__device__ __forceinline__ Big<NL> garner(const uint32_t r[K], const CarryArgs& A) {
+// mod p_j, with u_0 taken in (-p_0/2, p_0/2], then Horner over the primes,
+// the accumulator growing by one limb per step (after step j it holds a
+// signed value below 2^{32 (j+1) - 1} in magnitude).  Result sign-extended
+// to L >= K limbs.
+template <int K, int L>
+__device__ __forceinline__ void garner(const uint32_t (&r)[K], const CarryArgs& A, uint32_t (&acc)[L]) {
   uint32_t u[K];
 #pragma unroll
   for (int j = K - 1; j >= 0; --j) {
@@ -446,15 +444,38 @@ __device__ __forceinline__ Big<NL> garner(const uint32_t r[K], const CarryArgs& 
     u[j] = v;
   }
   const int32_t i0 = (u[0] > (A.p[0] >> 1)) ? int32_t(u[0] - A.p[0]) : int32_t(u[0]);
-  Big<NL> acc;
-  big_set_i32(acc, i0);
+  acc[0] = uint32_t(i0);
+  uint32_t sg = uint32_t(i0 >> 31);
 #pragma unroll
-  for (int j = 1; j < K; ++j) big_mad(acc, A.p[j], u[j]);
-  return acc;
+  for (int j = 1; j < K; ++j) {
+    const uint32_t pj = A.p[j];
+    uint64_t c = u[j];
+#pragma unroll
+    for (int i = 0; i < j; ++i) {
+      const uint64_t t = uint64_t(acc[i]) * pj + c;
+      acc[i] = uint32_t(t);
+      c = t >> 32;
+    }
+    acc[j] = sg * pj + uint32_t(c);
+    sg = uint32_t(int32_t(acc[j]) >> 31);
+  }
+#pragma unroll
+  for (int i = K; i < L; ++i) acc[i] = sg;
+}
+
+// Residue modulo p_k of a digit of magnitude m1 2^32 + m0 < 2^64 (m1 <= 2^31)
+// and sign neg: u = m 2^-32 mod p by one Montgomery reduction (m - (m0 p^-1
+// mod 2^32) p is a multiple of 2^32), then u * (+-2^64) * 2^-32.
+__device__ __forceinline__ uint32_t digit_residue64(uint32_t m0, uint32_t m1, bool neg, int k, const CarryArgs& A) {
+  const uint32_t p = A.p[k];
+  const uint32_t u = submod(m1, mulhi32(m0 * A.q[k], p), p);
+  const uint32_t w = neg ? A.c32n[k] : A.c32[k];
+  const uint32_t wq = neg ? A.c32nq[k] : A.c32q[k];
+  return mont_c(u, w, wq, p);
 }
 
 // Residue of a signed digit (|d| < 2^95) modulo p_k.
-__device__ __forceinline__ uint32_t digit_residue(const uint32_t d[DIGIT_LIMBS], int k, const CarryArgs& A) {
+__device__ __forceinline__ uint32_t digit_residue_wide(const uint32_t (&d)[DIGIT_LIMBS], int k, const CarryArgs& A) {
   const uint32_t p = A.p[k];
   const bool neg = int32_t(d[2]) < 0;
   uint32_t a0 = d[0], a1 = d[1], a2 = d[2];
@@ -474,10 +495,48 @@ __device__ __forceinline__ uint32_t digit_residue(const uint32_t d[DIGIT_LIMBS],
   return neg ? submod(0u, r, p) : r;
 }
 
-// Phase 1 for block b: digits with zero carry-in and the carry-out.
-template <int K, int NL>
-__device__ __forceinline__ void carry_block_local(const CarryJob& J, const CarryArgs& A, size_t n, size_t b,
-                                                  uint32_t dig[CARRY_L][DIGIT_LIMBS], Big<NL>& f) {
+// K residues of CARRY_L digits, written as uint4 per plane at dst + i0.
+template <int K>
+__device__ __forceinline__ void store_residues(const uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], const CarryArgs& A,
+                                               uint32_t* dst, size_t n, size_t i0) {
+  if (A.cbits <= 64) {
+    // |d| <= 2^(cbits-1) + 1 < 2^64: the low 64 bits give the magnitude.
+    uint32_t m0[CARRY_L], m1[CARRY_L];
+    bool neg[CARRY_L];
+#pragma unroll
+    for (int j = 0; j < CARRY_L; ++j) {
+      neg[j] = int32_t(dig[j][2]) < 0;
+      const uint64_t lo = uint64_t(dig[j][0]) | (uint64_t(dig[j][1]) << 32);
+      const uint64_t m = neg[j] ? (0ull - lo) : lo;
+      m0[j] = uint32_t(m);
+      m1[j] = uint32_t(m >> 32);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uint4 v;
+      v.x = digit_residue64(m0[0], m1[0], neg[0], k, A);
+      v.y = digit_residue64(m0[1], m1[1], neg[1], k, A);
+      v.z = digit_residue64(m0[2], m1[2], neg[2], k, A);
+      v.w = digit_residue64(m0[3], m1[3], neg[3], k, A);
+      *reinterpret_cast<uint4*>(dst + size_t(k) * n + i0) = v;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uint4 v;
+      v.x = digit_residue_wide(dig[0], k, A);
+      v.y = digit_residue_wide(dig[1], k, A);
+      v.z = digit_residue_wide(dig[2], k, A);
+      v.w = digit_residue_wide(dig[3], k, A);
+      *reinterpret_cast<uint4*>(dst + size_t(k) * n + i0) = v;
+    }
+  }
+}
+
+// Phase 1 for block b: digits with zero carry-in and the carry-out f.
+template <int K, int L, int QS>
+__device__ __forceinline__ void carry_block_local(const CarryJob& J, const CarryArgs& A, size_t n, size_t b, uint32_t s,
+                                                  uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], uint32_t (&f)[L]) {
   const size_t i0 = b * CARRY_L;
   uint32_t res[K][CARRY_L];
 #pragma unroll
@@ -499,38 +558,56 @@ __device__ __forceinline__ void carry_block_local(const CarryJob& J, const Carry
       sd[l][3] = v.w;
     }
   }
-  big_set_i32(f, 0);
+#pragma unroll
+  for (int i = 0; i < L; ++i) f[i] = 0u;
 #pragma unroll
   for (int j = 0; j < CARRY_L; ++j) {
     uint32_t r[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) r[k] = res[k][j];
-    Big<NL> v = garner<K, NL>(r, A);
-    if (J.a != 1u) big_mad(v, J.a, 0u);
+    uint32_t v[L];
+    garner<K, L>(r, A, v);
+    if (J.a != 1u) big_mad<L>(v, J.a, 0u);
     if (J.sub != nullptr) {
       const uint32_t d[DIGIT_LIMBS] = {sd[0][j], sd[1][j], sd[2][j]};
-      big_sub(v, big_from_digit<NL>(d));
+      uint32_t e[L];
+      big_from_digit<L>(d, e);
+      big_sub<L>(v, e);
     }
-    big_add(f, v);
-    big_split(f, A.cbits, dig[j]);
+    big_add<L>(f, v);
+    big_split<L, QS>(f, s, dig[j]);
   }
 }
 
-// Phase 2: add the carry-in to the block's digits.
-template <int NL>
-__device__ __forceinline__ void carry_block_finish(uint32_t dig[CARRY_L][DIGIT_LIMBS], Big<NL> f, int cbits) {
+// Phase 2: add the carry-in f to the block's digits.
+template <int L, int QS>
+__device__ __forceinline__ void carry_block_finish(uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], uint32_t (&f)[L], uint32_t s) {
 #pragma unroll
   for (int j = 0; j < CARRY_L - 1; ++j) {
-    big_add(f, big_from_digit<NL>(dig[j]));
-    big_split(f, cbits, dig[j]);
+    uint32_t e[L];
+    big_from_digit<L>(dig[j], e);
+    big_add<L>(f, e);
+    big_split<L, QS>(f, s, dig[j]);
   }
-  big_add(f, big_from_digit<NL>(dig[CARRY_L - 1]));
+  uint32_t e[L];
+  big_from_digit<L>(dig[CARRY_L - 1], e);
+  big_add<L>(f, e);
 #pragma unroll
-  for (int l = 0; l < DIGIT_LIMBS; ++l) dig[CARRY_L - 1][l] = f.w[l];
+  for (int l = 0; l < DIGIT_LIMBS; ++l) dig[CARRY_L - 1][l] = f[l];
+}
+
+template <int L>
+__device__ __forceinline__ void big_neg(uint32_t (&f)[L]) {
+  uint32_t z[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) z[i] = 0u;
+  big_sub<L>(z, f);
+#pragma unroll
+  for (int i = 0; i < L; ++i) f[i] = z[i];
 }
 
 // d += v for a signed DIGIT_LIMBS digit and small signed v.
-__device__ __forceinline__ void digit_add(uint32_t d[DIGIT_LIMBS], int32_t v) {
+__device__ __forceinline__ void digit_add(uint32_t (&d)[DIGIT_LIMBS], int32_t v) {
   const uint64_t t0 = uint64_t(d[0]) + uint32_t(v);
   d[0] = uint32_t(t0);
   const uint32_t ext = uint32_t(v >> 31);
@@ -539,32 +616,19 @@ __device__ __forceinline__ void digit_add(uint32_t d[DIGIT_LIMBS], int32_t v) {
   d[2] = d[2] + ext + uint32_t(t1 >> 32);
 }
 
-// d += sgn * 2^cbits.
-__device__ __forceinline__ void digit_add_base(uint32_t d[DIGIT_LIMBS], int cbits, int sgn) {
+// d += 2^cbits.
+__device__ __forceinline__ void digit_add_base(uint32_t (&d)[DIGIT_LIMBS], int cbits) {
   const int qs = cbits >> 5, s = cbits & 31;
-  uint32_t b[DIGIT_LIMBS];
+  uint32_t c = 0;
 #pragma unroll
-  for (int i = 0; i < DIGIT_LIMBS; ++i) b[i] = (i == qs) ? (1u << s) : 0u;
-  if (sgn < 0) {
-    uint32_t br = 0;
-#pragma unroll
-    for (int i = 0; i < DIGIT_LIMBS; ++i) {
-      const uint64_t t = uint64_t(d[i]) - b[i] - br;
-      d[i] = uint32_t(t);
-      br = uint32_t(t >> 32) & 1u;
-    }
-  } else {
-    uint32_t c = 0;
-#pragma unroll
-    for (int i = 0; i < DIGIT_LIMBS; ++i) {
-      const uint64_t t = uint64_t(d[i]) + b[i] + c;
-      d[i] = uint32_t(t);
-      c = uint32_t(t >> 32);
-    }
+  for (int i = 0; i < DIGIT_LIMBS; ++i) {
+    const uint64_t t = uint64_t(d[i]) + ((i == qs) ? (1u << s) : 0u) + c;
+    d[i] = uint32_t(t);
+    c = uint32_t(t >> 32);
   }
 }
 
-__device__ __forceinline__ int digit_sign(const uint32_t d[DIGIT_LIMBS]) {
+__device__ __forceinline__ int digit_sign(const uint32_t (&d)[DIGIT_LIMBS]) {
   if ((d[0] | d[1] | d[2]) == 0u) return 0;
   return (int32_t(d[2]) < 0) ? -1 : 1;
 }
@@ -572,21 +636,16 @@ __device__ __forceinline__ int digit_sign(const uint32_t d[DIGIT_LIMBS]) {
 // Sign of the whole number: scan blocks from the top, finishing each with the
 // zero-in carry of its predecessor, until a nonzero digit appears (the top
 // digit is nonzero except with probability ~2^-cbits per block).
-template <int K, int NL>
-__device__ int number_sign(const CarryJob& J, const CarryArgs& A, size_t n) {
+template <int K, int L, int QS>
+__device__ int number_sign(const CarryJob& J, const CarryArgs& A, size_t n, uint32_t s) {
   const size_t nb = n / CARRY_L;
   for (size_t b = nb; b-- > 0;) {
     uint32_t d[CARRY_L][DIGIT_LIMBS], pd[CARRY_L][DIGIT_LIMBS];
-    Big<NL> f, fi;
-    carry_block_local<K, NL>(J, A, n, b, d, f);
-    carry_block_local<K, NL>(J, A, n, (b == 0) ? nb - 1 : b - 1, pd, fi);
-    if (b == 0 && J.nega) {
-      Big<NL> z;
-      big_set_i32(z, 0);
-      big_sub(z, fi);
-      fi = z;
-    }
-    carry_block_finish<NL>(d, fi, A.cbits);
+    uint32_t f[L], fi[L];
+    carry_block_local<K, L, QS>(J, A, n, b, s, d, f);
+    carry_block_local<K, L, QS>(J, A, n, (b == 0) ? nb - 1 : b - 1, s, pd, fi);
+    if (b == 0 && J.nega) big_neg<L>(fi);
+    carry_block_finish<L, QS>(d, fi, s);
 #pragma unroll
     for (int j = CARRY_L - 1; j >= 0; --j) {
       const int sg = digit_sign(d[j]);
@@ -596,46 +655,42 @@ __device__ int number_sign(const CarryJob& J, const CarryArgs& A, size_t n) {
   return 0;
 }
 
-template <int K>
+// Grid: (ceil(nb / (CARRY_WG - 1)), jobs), nb = n / CARRY_L blocks.
+template <int K, int QS>
 __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
-  constexpr int NL = K + 1;
-  __shared__ Big<NL> cl[CARRY_WG];
+  constexpr int L = K + 1;
+  __shared__ uint32_t cl[L][CARRY_WG];
   const CarryJob& J = A.job[blockIdx.y];
   const size_t n = size_t(1) << A.LN;
   const size_t nb = n / CARRY_L;
   const uint32_t t = threadIdx.x;
-  const size_t b = size_t(blockIdx.x) * blockDim.x + t;
-  const bool active = b < nb;
+  const uint32_t s = uint32_t(A.cbits) & 31u;
+  const size_t g = size_t(blockIdx.x) * (CARRY_WG - 1) + t;  // owned block + 1
+  const size_t b = (g == 0) ? nb - 1 : g - 1;
+  const bool active = g <= nb;
 
   uint32_t dig[CARRY_L][DIGIT_LIMBS];
-  Big<NL> fo;
-  if (active) carry_block_local<K, NL>(J, A, n, b, dig, fo);
-  cl[t] = fo;
-  __syncthreads();
-  if (!active) return;
-
-  Big<NL> fin;
-  if (t > 0) {
-    fin = cl[t - 1];
-  } else {
-    uint32_t pd[CARRY_L][DIGIT_LIMBS];
-    const size_t pb = (b == 0) ? nb - 1 : b - 1;
-    carry_block_local<K, NL>(J, A, n, pb, pd, fin);
-    if (b == 0 && J.nega) {
-      Big<NL> z;
-      big_set_i32(z, 0);
-      big_sub(z, fin);
-      fin = z;
-    }
+  uint32_t fo[L];
+  if (active) {
+    carry_block_local<K, L, QS>(J, A, n, b, s, dig, fo);
+#pragma unroll
+    for (int i = 0; i < L; ++i) cl[i][t] = fo[i];
   }
-  carry_block_finish<NL>(dig, fin, A.cbits);
+  __syncthreads();
+  if (!active || t == 0) return;
+
+  uint32_t fin[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) fin[i] = cl[i][t - 1];
+  if (b == 0 && J.nega) big_neg<L>(fin);
+  carry_block_finish<L, QS>(dig, fin, s);
 
   if (J.canon == 1 && (b == 0 || b == nb - 1)) {
     // m' into [0, R): both ends of the number take the same decision, adding
     // R = B^n - 1 to a negative value; then m' + 2.  (mod Q needs nothing: the
     // result is already the unique member of its class in (-B^n/2, B^n/2).)
-    const int sg = number_sign<K, NL>(J, A, n);
-    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits, 1);
+    const int sg = number_sign<K, L, QS>(J, A, n, s);
+    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits);
     if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
   }
 
@@ -651,27 +706,27 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
       *reinterpret_cast<uint4*>(J.dst + size_t(l) * n + i0) = v;
     }
   } else {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      uint4 v;
-      v.x = digit_residue(dig[0], k, A);
-      v.y = digit_residue(dig[1], k, A);
-      v.z = digit_residue(dig[2], k, A);
-      v.w = digit_residue(dig[3], k, A);
-      *reinterpret_cast<uint4*>(J.dst + size_t(k) * n + i0) = v;
-    }
+    store_residues<K>(dig, A, J.dst, n, i0);
   }
 }
 
-// Raw signed digits -> K residue planes.
+// Raw signed digits -> K residue planes (CARRY_L digits per thread).
 template <int K>
 __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const CarryArgs A) {
   const size_t n = size_t(1) << A.LN;
-  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t d[DIGIT_LIMBS] = {digits[i], digits[n + i], digits[2 * n + i]};
+  const size_t b = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= n / CARRY_L) return;
+  const size_t i0 = b * CARRY_L;
+  uint32_t dig[CARRY_L][DIGIT_LIMBS];
 #pragma unroll
-  for (int k = 0; k < K; ++k) dst[size_t(k) * n + i] = digit_residue(d, k, A);
+  for (int l = 0; l < DIGIT_LIMBS; ++l) {
+    const uint4 v = *reinterpret_cast<const uint4*>(digits + size_t(l) * n + i0);
+    dig[0][l] = v.x;
+    dig[1][l] = v.y;
+    dig[2][l] = v.z;
+    dig[3][l] = v.w;
+  }
+  store_residues<K>(dig, A, dst, n, i0);
 }
 
 // Explicit instantiations, looked up by the host engine: row lines of 2^5 ..
@@ -699,10 +754,14 @@ ANNIE_COL(8)
 ANNIE_COL(9)
 ANNIE_COL(10)
 ANNIE_COL(11)
-template __global__ void k_carry<2>(const CarryArgs);
-template __global__ void k_carry<3>(const CarryArgs);
-template __global__ void k_carry<4>(const CarryArgs);
-template __global__ void k_carry<5>(const CarryArgs);
+#define ANNIE_CARRY(K)                                 \
+  template __global__ void k_carry<K, 0>(const CarryArgs); \
+  template __global__ void k_carry<K, 1>(const CarryArgs); \
+  template __global__ void k_carry<K, 2>(const CarryArgs);
+ANNIE_CARRY(2)
+ANNIE_CARRY(3)
+ANNIE_CARRY(4)
+ANNIE_CARRY(5)
 template __global__ void k_digits_to_res<2>(const uint32_t*, uint32_t*, const CarryArgs);
 template __global__ void k_digits_to_res<3>(const uint32_t*, uint32_t*, const CarryArgs);
 template __global__ void k_digits_to_res<4>(const uint32_t*, uint32_t*, const CarryArgs);
