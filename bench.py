@@ -73,6 +73,27 @@ def alg_bytes(kind, U):
     return jobs * 2 * U
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "square_mul_40M_digits_pmc.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` and its VALU utilisation from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (counters scaled by a plain copy of known size), or (None, None)."""
+    try:
+        lines = open(PMC_SUMMARY).read().splitlines()
+    except OSError:
+        return None, None
+    for ln in lines:
+        f = ln.split()
+        if len(f) < 2 or f[0] != "traffic" or f[1] != kernel:
+            continue
+        kv = dict(x.split("=", 1) for x in f[2:])
+        b = (float(kv["fetch_kib"]) * float(kv["fetch_factor"]) + float(kv["write_kib"]) * float(kv["write_factor"])) * 1024
+        return int(round(b)), float(kv["valu_util"]) if "valu_util" in kv else None
+    return None, None
+
+
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -172,6 +193,7 @@ def main():
     avg_ms = tot / cnt
     achieved = alg_bytes(dom, U) / (avg_ms * 1e-3) / 1e9
     b_alg_sq = 27 * U  # (16P - 5) U with P = 2 two-pass directions at this length
+    traffic, valu_util = measured_traffic(dom)
 
     # A plain register copy of known size (U bytes read, U written) for
     # calibrating HBM byte counters in profiling runs.
@@ -221,9 +243,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic is not None else None,
             "alg_bytes_per_launch": alg_bytes(dom, U),
             "avg_launch_ms": round(avg_ms, 5),
+            "valu_util": valu_util,
         },
     }
     if cpu_res is not None and "ms_per_squaring" in cpu_res:
