@@ -81,6 +81,15 @@ def load_library(path=HIP_LIB_PATH):
     u32p = ctypes.POINTER(ctypes.c_uint32)
     lib.annie_hip_plan_constants.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, sz]
     lib.annie_hip_twiddles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, u32p, sz]
+    lib.annie_hip_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, ctypes.c_int,
+                                           ctypes.POINTER(h)]
+    lib.annie_hip_batch_load.argtypes = [h, sz, u64p, sz, ctypes.c_int32, u64p, sz, ctypes.POINTER(sz)]
+    lib.annie_hip_batch_set1.argtypes = [h, u32p, sz]
+    lib.annie_hip_batch_copy.argtypes = [h, sz, sz, u32p, sz]
+    lib.annie_hip_batch_square_mul.argtypes = [h, u32p, u32p, sz]
+    lib.annie_hip_batch_mul.argtypes = [h, sz, sz, u32p, sz]
+    lib.annie_hip_batch_get.argtypes = [h, sz, u32p, sz, u64p, sz, ctypes.POINTER(sz)]
+    lib.annie_hip_batch_sync.argtypes = [h]
     lib.annie_hip_destroy.argtypes = [h]
     lib.annie_hip_destroy.restype = None
     lib.annie_hip_last_error.restype = ctypes.c_char_p
@@ -321,3 +330,65 @@ def request_quit():
 
 def clear_quit():
     load_host_library().annie_clear_quit()
+
+
+class BatchTest(ctypes.Structure):
+    _fields_ = [("N", ctypes.POINTER(ctypes.c_uint64)), ("nlimbs", ctypes.c_size_t), ("is_euler", ctypes.c_int),
+                ("a_ref", ctypes.c_int32), ("arrival", ctypes.c_double)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("engine", ctypes.c_int), ("device", ctypes.c_int), ("max_slots", ctypes.c_size_t),
+                ("hbm_budget_mb", ctypes.c_double), ("realtime", ctypes.c_int), ("time_limit", ctypes.c_double),
+                ("verbose", ctypes.c_int)]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("is_prp", ctypes.c_int), ("res64", ctypes.c_uint64),
+                ("gl_ok", ctypes.c_int), ("a", ctypes.c_int32), ("iterations", ctypes.c_long),
+                ("arrival", ctypes.c_double), ("admitted", ctypes.c_double), ("finished", ctypes.c_double),
+                ("message", ctypes.c_char * 256)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("squarings", ctypes.c_double), ("classes", ctypes.c_int),
+                ("slots", ctypes.c_size_t)]
+
+
+def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
+              verbose=False):
+    """Run many tests together, one engine batch per length class, with
+    continuous admission (annie_batch_run in include/annie_host.h).
+
+    `tests`: iterable of dicts {"N": int, "is_euler": bool, "a": int (Fermat
+    base, 0 = 2), "arrival": seconds}.  engine "gpu" is the gfx950 engine;
+    "gmp" runs the same driver on exact GMP transforms on the CPU.
+    Returns (results, stats): one dict per test (status, is_prp, res64, gl_ok,
+    a, iterations, arrival, admitted, finished, message) and a stats dict.
+    """
+    lib = load_host_library()
+    if not hasattr(lib, "_batch_ready"):
+        lib.annie_batch_run.argtypes = [ctypes.POINTER(BatchTest), ctypes.c_size_t, ctypes.POINTER(BatchOpts),
+                                        ctypes.POINTER(BatchResult), ctypes.POINTER(BatchStats)]
+        lib._batch_ready = True
+    tests = list(tests)
+    keep = []
+    arr = (BatchTest * max(len(tests), 1))()
+    for i, t in enumerate(tests):
+        limbs, n = int_to_limbs(int(t["N"]))
+        keep.append(limbs)
+        arr[i] = BatchTest(limbs, n, 1 if t.get("is_euler") else 0, int(t.get("a", 0)), float(t.get("arrival", 0.0)))
+    opts = BatchOpts(0 if engine == "gpu" else 1, device, max_slots, hbm_budget_mb, 1 if realtime else 0,
+                     time_limit, 1 if verbose else 0)
+    res = (BatchResult * max(len(tests), 1))()
+    st = BatchStats()
+    rc = lib.annie_batch_run(arr, len(tests), ctypes.byref(opts), res, ctypes.byref(st))
+    out = []
+    for i in range(len(tests)):
+        r = res[i]
+        out.append({"status": STATUS.get(r.status, r.status), "is_prp": bool(r.is_prp), "res64": r.res64,
+                    "gl_ok": bool(r.gl_ok), "a": r.a, "iterations": r.iterations, "arrival": r.arrival,
+                    "admitted": r.admitted, "finished": r.finished,
+                    "message": r.message.decode(errors="replace")})
+    stats = {"rc": rc, "seconds": st.seconds, "squarings": st.squarings, "classes": st.classes, "slots": st.slots}
+    return out, stats

@@ -10,6 +10,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -32,6 +33,8 @@ template <int K, int QS>
 __global__ void k_carry(const CarryArgs A);
 template <int K>
 __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const CarryArgs A);
+__global__ void k_copy_slots(uint32_t* base, size_t dst_off, size_t src_off, size_t len, const uint32_t* slot_ids,
+                             size_t slot_stride);
 
 static thread_local std::string g_last_error;
 
@@ -59,14 +62,44 @@ enum Reg : int {
   NREG
 };
 
+struct Mpz {
+  mpz_t v;
+  Mpz() { mpz_init(v); }
+  ~Mpz() { mpz_clear(v); }
+  Mpz(const Mpz&) = delete;
+  Mpz& operator=(const Mpz&) = delete;
+};
+
+// Ring of pinned staging buffers for the per-operation slot lists and
+// multipliers of a batch (entry e is rewritten only after its previous
+// host-to-device copy has completed).
+constexpr int NSTAGE = 16;
+struct Stage {
+  uint32_t* dev = nullptr;
+  uint32_t* host = nullptr;
+  size_t cap = 0;  // words per entry
+  int pos = 0;
+  hipEvent_t ev[NSTAGE] = {};
+  bool pending[NSTAGE] = {};
+};
+
 struct Ctx {
   Plan pl;
   int device = 0;
   int m1 = 0, m2 = 0;
   size_t n = 0;
   hipStream_t stream = nullptr;
-  uint32_t* regs = nullptr;   // NREG * K * n words
-  uint32_t* raw = nullptr;    // 2 * DIGIT_LIMBS * n words: T_Q digits, get() digits
+  // One allocation of nslots_alloc slots; a slot is NREG registers of K * n
+  // words followed by 2 * DIGIT_LIMBS * n words of raw digits (T_Q, get()).
+  uint32_t* mem = nullptr;
+  size_t nslots_alloc = 1, slot_stride = 0;
+  uint32_t* regs = nullptr;   // slot 0 registers
+  uint32_t* raw = nullptr;    // slot 0 raw digits
+  // Slot list of the operation in progress (device array, null: slot 0 only).
+  const uint32_t* slot_ids = nullptr;
+  int nslots = 1;
+  Stage stage;
+  std::vector<std::unique_ptr<Mpz>> slotN;  // modulus of each slot (batches)
   TwDev* tw = nullptr;        // 4 tables * K * n roots (Montgomery form)
   mpz_t N, factor;
   bool init_failed = false;
@@ -159,6 +192,9 @@ static void launch_col_m(bool inverse, dim3 g, dim3 b, size_t shm, hipStream_t s
 static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   NttArgs A = c.nbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
+  A.slot_ids = c.slot_ids;
+  A.slot_stride = c.slot_stride;
+  A.njobs = nj;
   A.m = c.m2;
   A.m1 = c.m1;
   const uint32_t S = 1u << c.m2, Tl = S / NTT_R;
@@ -166,7 +202,7 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   uint32_t lines = (Tl >= 64) ? 1u : 64u / Tl;
   if (lines > rows) lines = rows;
   A.cols = int(lines);
-  const dim3 grid(rows / lines, c.pl.K, nj), block(lines * Tl);
+  const dim3 grid(rows / lines, c.pl.K, nj * c.nslots), block(lines * Tl);
   const size_t shm = size_t(lines) * (S + (S >> 5)) * sizeof(uint32_t);
   ProfScope ps(c, "k_row<" + std::to_string(c.m2) + "," + row_mode_name(mode) + ">x" + std::to_string(nj));
   switch (c.m2) {
@@ -188,6 +224,9 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
 static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   NttArgs A = c.nbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
+  A.slot_ids = c.slot_ids;
+  A.slot_stride = c.slot_stride;
+  A.njobs = nj;
   A.m = c.m1;
   A.m1 = c.m1;
   const uint32_t S = 1u << c.m1, Tl = S / NTT_R;
@@ -198,7 +237,7 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   if (C > 16) C = 16;
   if (C > n2) C = n2;
   A.cols = int(C);
-  const dim3 grid(n2 / C, c.pl.K, nj), block(C * Tl);
+  const dim3 grid(n2 / C, c.pl.K, nj * c.nslots), block(C * Tl);
   const size_t shm = size_t(C) * S * sizeof(uint32_t);
   ProfScope ps(c, std::string(inverse ? "k_col_inv<" : "k_col_fwd<") + std::to_string(c.m1) + ">x" + std::to_string(nj));
   switch (c.m1) {
@@ -244,12 +283,16 @@ static int convolve(Ctx& c, int mode, const NttJob* jobs, int nj) {
   return launch_row(c, mode, jobs, nj);
 }
 
-static int launch_carry(Ctx& c, const CarryJob* jobs, int nj) {
+static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_slot = nullptr) {
   CarryArgs A = c.cbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
+  A.slot_ids = c.slot_ids;
+  A.slot_stride = c.slot_stride;
+  A.njobs = nj;
+  A.a_slot = a_slot;
   const size_t nb = c.n / CARRY_L;
   // CARRY_WG - 1 owned blocks per workgroup (thread 0 only hands over a carry).
-  const dim3 grid(unsigned((nb + CARRY_WG - 2) / (CARRY_WG - 1)), nj), block(CARRY_WG);
+  const dim3 grid(unsigned((nb + CARRY_WG - 2) / (CARRY_WG - 1)), nj * c.nslots), block(CARRY_WG);
   const int qs = c.pl.cbits() >> 5;
   if (qs > 2) return fail(ANNIE_ERR_SIZE, "digit width above 95 bits");
   ProfScope ps(c, "k_carry<" + std::to_string(c.pl.K) + ">x" + std::to_string(nj));
@@ -302,13 +345,14 @@ static int reduce(Ctx& c, uint32_t* dst, bool raw_out) {
   return launch_carry(c, &cj, 1);
 }
 
-// squareMMGW (src/transformGPU.h:556-568).
-static int square_mul(Ctx& c, uint32_t a) {
+// squareMMGW (src/transformGPU.h:556-568).  a_slot: per-slot multipliers of a
+// batch operation (device array in slot-list order), replacing a.
+static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
   int rc;
   const NttJob j[2] = {job(c.reg(R0), c.reg(RWR), c.fwdR(), c.invR()), job(c.reg(R0), c.reg(RWQ), c.fwdQ(), c.invQ())};
   if ((rc = convolve(c, ROW_SQR, j, 2))) return rc;
   const CarryJob cj[2] = {cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0), cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1)};
-  if ((rc = launch_carry(c, cj, 2))) return rc;
+  if ((rc = launch_carry(c, cj, 2, a_slot))) return rc;
   return reduce(c, c.reg(R0), false);
 }
 
@@ -327,7 +371,15 @@ static int mul(Ctx& c, int dst, int src) {
 
 static int copy_reg(Ctx& c, int dst, int src) {
   if (dst == src) return ANNIE_OK;
-  HIPCHK(hipMemcpyAsync(c.reg(dst), c.reg(src), size_t(c.pl.K) * c.n * 4, hipMemcpyDeviceToDevice, c.stream));
+  const size_t len = size_t(c.pl.K) * c.n;
+  if (c.slot_ids == nullptr) {
+    HIPCHK(hipMemcpyAsync(c.reg(dst), c.reg(src), len * 4, hipMemcpyDeviceToDevice, c.stream));
+    return ANNIE_OK;
+  }
+  const unsigned blocks = unsigned(std::min<size_t>((len / 4 + 255) / 256, 64));
+  hipLaunchKernelGGL(k_copy_slots, dim3(blocks, c.nslots), dim3(256), 0, c.stream, c.mem, size_t(dst) * len,
+                     size_t(src) * len, len, c.slot_ids, c.slot_stride);
+  HIPCHK(hipGetLastError());
   return ANNIE_OK;
 }
 
@@ -401,36 +453,58 @@ static void digits_to_mpz(const Ctx& c, const std::vector<uint32_t>& d, mpz_t z)
   }
 }
 
-static int upload_mpz(Ctx& c, int r, const mpz_t x) {
+static uint32_t* slot_reg(Ctx& c, int r, size_t slot) { return c.reg(r) + slot * c.slot_stride; }
+
+static int upload_mpz(Ctx& c, int r, const mpz_t x, size_t slot = 0) {
   std::vector<uint32_t> h;
   mpz_to_residues(c, x, h);
-  HIPCHK(hipMemcpyAsync(c.reg(r), h.data(), h.size() * 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(slot_reg(c, r, slot), h.data(), h.size() * 4, hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   return ANNIE_OK;
 }
 
-static int scale_reg(Ctx& c, int r) {
+static int scale_reg(Ctx& c, int r, size_t slot = 0) {
   uint32_t s[KMAX] = {0, 0, 0, 0, 0};
   for (int k = 0; k < c.pl.K; ++k) s[k] = c.pc.sq_scale[k];  // mont(x, n^-1 2^64) = x n^-1 2^32
   const dim3 grid(unsigned((c.n + 255) / 256), c.pl.K), block(256);
-  hipLaunchKernelGGL(k_scale, grid, block, 0, c.stream, c.reg(r), c.n, c.pl.K, c.nbase, s[0], s[1], s[2], s[3], s[4]);
+  hipLaunchKernelGGL(k_scale, grid, block, 0, c.stream, slot_reg(c, r, slot), c.n, c.pl.K, c.nbase, s[0], s[1], s[2],
+                     s[3], s[4]);
   HIPCHK(hipGetLastError());
   return ANNIE_OK;
 }
 
-static int get_value(Ctx& c, int src, mpz_t z) {
+// get() for every slot of the current list (slot 0 without a list): T_Q = src
+// as raw digits (a carry pass over its own residues), T_R = src, reduce to
+// raw digits, read back, digits -> integer mod N.  mods[i] is the modulus of
+// list position i; zs[i] receives the value.
+static int get_values(Ctx& c, int src, const size_t* slots, const mpz_t* const* mods, mpz_t* const* zs, size_t count) {
   int rc;
-  // T_Q = src as raw digits (a carry pass over its own residues), T_R = src.
   const CarryJob cj = cjob(c.reg(src), c.tq(), 1u, 1, 0, 1);
   if ((rc = launch_carry(c, &cj, 1))) return rc;
   if ((rc = copy_reg(c, RTR, src))) return rc;
   if ((rc = reduce(c, c.rawtmp(), true))) return rc;
-  std::vector<uint32_t> d(size_t(DIGIT_LIMBS) * c.n);
-  HIPCHK(hipMemcpyAsync(d.data(), c.rawtmp(), d.size() * 4, hipMemcpyDeviceToHost, c.stream));
+  const size_t w = size_t(DIGIT_LIMBS) * c.n;
+  std::vector<uint32_t> d(w * count);
+  for (size_t i = 0; i < count; ++i)
+    HIPCHK(hipMemcpyAsync(d.data() + i * w, c.rawtmp() + slots[i] * c.slot_stride, w * 4, hipMemcpyDeviceToHost,
+                          c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
-  digits_to_mpz(c, d, z);
-  mpz_mod(z, z, c.N);
+  for (size_t i = 0; i < count; ++i) {
+    const std::vector<uint32_t> di(d.begin() + i * w, d.begin() + (i + 1) * w);
+    digits_to_mpz(c, di, *zs[i]);
+    mpz_mod(*zs[i], *zs[i], *mods[i]);
+  }
   return ANNIE_OK;
+}
+
+static int get_value(Ctx& c, int src, mpz_t z) {
+  const size_t slot0 = 0;
+  const mpz_t* m = &c.N;
+  Mpz v;
+  mpz_t* zp = &v.v;
+  const int rc = get_values(c, src, &slot0, &m, &zp, 1);
+  if (rc == ANNIE_OK) mpz_set(z, v.v);
+  return rc;
 }
 
 static void limbs_to_mpz(mpz_t z, const uint64_t* l, size_t nl) { mpz_import(z, nl, -1, sizeof(uint64_t), 0, 0, l); }
@@ -459,58 +533,42 @@ static void destroy(annie_hip_t* h) {
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
   }
-  if (c.regs) (void)hipFree(c.regs);
-  if (c.raw) (void)hipFree(c.raw);
+  if (c.mem) (void)hipFree(c.mem);
   if (c.tw) (void)hipFree(c.tw);
+  if (c.stage.dev) (void)hipFree(c.stage.dev);
+  if (c.stage.host) (void)hipHostFree(c.stage.host);
+  for (hipEvent_t e : c.stage.ev)
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : c.ev) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   mpz_clears(c.N, c.factor, nullptr);
   delete h;
 }
 
-static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_ln, annie_hip_t** out) {
-  if (!out || !Nl || nl == 0) return fail(ANNIE_ERR_ARG, "null argument");
-  *out = nullptr;
-  if (a < 1) return fail(ANNIE_ERR_ARG, "base must be positive");
-  std::unique_ptr<annie_hip, void (*)(annie_hip_t*)> h(new annie_hip(), destroy);
-  Ctx& c = h->c;
-  mpz_inits(c.N, c.factor, nullptr);
-  limbs_to_mpz(c.N, Nl, nl);
-  if (mpz_even_p(c.N) || mpz_cmp_ui(c.N, 3) < 0) return fail(ANNIE_ERR_ARG, "N must be odd and >= 3");
-  c.pl = plan_for(c.N, a);
-  if (c.pl.LN < min_ln) return fail(ANNIE_ERR_SIZE, "NTT size < 2^" + std::to_string(min_ln) + " is not supported");
-  if (c.pl.LN > 23) return fail(ANNIE_ERR_SIZE, "NTT size > 2^23 is not supported");
-  if (c.pl.LN < 5) return fail(ANNIE_ERR_SIZE, "NTT size < 2^5 is not supported by this engine");
+// R = B^n - 1 of a plan.
+static void plan_R(const Plan& pl, mpz_t R) {
+  mpz_set_ui(R, 1);
+  mpz_mul_2exp(R, R, mp_bitcnt_t(pl.cbits()) << pl.LN);
+  mpz_sub_ui(R, R, 1);
+}
+
+// Device state shared by the slots of a context: stream, register file of
+// `slots` slots, twiddle tables, kernel constants.
+static int setup_device(Ctx& c, int device, size_t slots) {
   c.n = c.pl.n();
   c.m1 = (c.pl.LN <= 13) ? 0 : c.pl.LN / 2;
   c.m2 = c.pl.LN - c.m1;
   c.device = device;
-
-  // R = B^n - 1; N must be invertible mod R (src/transformGPU.h:652-661).
-  mpz_t R, t;
-  mpz_inits(R, t, nullptr);
-  mpz_set_ui(R, 1);
-  mpz_mul_2exp(R, R, mp_bitcnt_t(c.pl.cbits()) << c.pl.LN);
-  mpz_sub_ui(R, R, 1);
-  struct Clear {
-    mpz_t& a;
-    mpz_t& b;
-    ~Clear() { mpz_clears(a, b, nullptr); }
-  } clear_rt{R, t};
-  if (mpz_invert(t, c.N, R) == 0) {
-    mpz_gcd(c.factor, c.N, R);
-    c.init_failed = true;
-    *out = h.release();
-    return ANNIE_INIT_FAILED;
-  }
-
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   const size_t K = size_t(c.pl.K), n = c.n;
-  if (hipMalloc(&c.regs, size_t(NREG) * K * n * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "register file");
-  if (hipMalloc(&c.raw, 2 * size_t(DIGIT_LIMBS) * n * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "digit buffers");
+  c.nslots_alloc = slots;
+  c.slot_stride = size_t(NREG) * K * n + 2 * size_t(DIGIT_LIMBS) * n;
+  if (hipMalloc(&c.mem, slots * c.slot_stride * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "register file");
+  c.regs = c.mem;
+  c.raw = c.mem + size_t(NREG) * K * n;
   if (hipMalloc(&c.tw, 4 * K * n * sizeof(TwDev)) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "twiddles");
-  HIPCHK(hipMemsetAsync(c.regs, 0, size_t(NREG) * K * n * 4, c.stream));
+  HIPCHK(hipMemsetAsync(c.mem, 0, slots * c.slot_stride * 4, c.stream));
 
   c.pc = make_consts(c.pl);
   for (int k = 0; k < KMAX; ++k) {
@@ -531,45 +589,147 @@ static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_
   }
   c.nbase.LN = c.cbase.LN = c.pl.LN;
   c.cbase.cbits = c.pl.cbits();
+  c.nbase.njobs = c.cbase.njobs = 1;
 
-  // Twiddle tables.
-  {
-    std::vector<TwDev> host(4 * K * n);
-    for (size_t k = 0; k < K; ++k) {
-      const PrimeTables pt = make_tables(c.pl.LN, int(k));
-      for (size_t i = 0; i < n; ++i) {
-        host[(0 * K + k) * n + i] = pt.fwdR[i].w;
-        host[(1 * K + k) * n + i] = pt.invR[i].w;
-        host[(2 * K + k) * n + i] = pt.fwdQ[i].w;
-        host[(3 * K + k) * n + i] = pt.invQ[i].w;
-      }
+  std::vector<TwDev> host(4 * K * n);
+  for (size_t k = 0; k < K; ++k) {
+    const PrimeTables pt = make_tables(c.pl.LN, int(k));
+    for (size_t i = 0; i < n; ++i) {
+      host[(0 * K + k) * n + i] = pt.fwdR[i].w;
+      host[(1 * K + k) * n + i] = pt.invR[i].w;
+      host[(2 * K + k) * n + i] = pt.fwdQ[i].w;
+      host[(3 * K + k) * n + i] = pt.invQ[i].w;
     }
-    HIPCHK(hipMemcpyAsync(c.tw, host.data(), host.size() * sizeof(TwDev), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipStreamSynchronize(c.stream));
   }
-
-  int rc;
-  // N_Q: negacyclic transform of N; Ninv_R: cyclic transform of N^-1 mod R
-  // (src/transformGPU.h:737-741), both scaled by n^-1 2^32 for the pointwise step.
-  if ((rc = upload_mpz(c, RTMP, c.N))) return rc;
-  {
-    NttJob j = job(c.reg(RTMP), c.reg(RNQ), c.fwdQ(), c.invQ());
-    if ((rc = forward(c, &j, 1))) return rc;
-    if ((rc = scale_reg(c, RNQ))) return rc;
-  }
-  if ((rc = upload_mpz(c, RTMP, t))) return rc;
-  {
-    NttJob j = job(c.reg(RTMP), c.reg(RNIR), c.fwdR(), c.invR());
-    if ((rc = forward(c, &j, 1))) return rc;
-    if ((rc = scale_reg(c, RNIR))) return rc;
-  }
-  // The MMGW form of 1 is R/2 mod N (src/transformGPU.h:743-747).
-  mpz_mod(t, R, c.N);
-  if (mpz_odd_p(t)) mpz_add(t, t, c.N);
-  mpz_fdiv_q_2exp(t, t, 1);
-  if ((rc = upload_mpz(c, RONE, t))) return rc;
+  HIPCHK(hipMemcpyAsync(c.tw, host.data(), host.size() * sizeof(TwDev), hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  return ANNIE_OK;
+}
+
+// Per-modulus constants of one slot: N_Q, the negacyclic transform of N, and
+// Ninv_R, the cyclic transform of N^-1 mod R (src/transformGPU.h:737-741),
+// both scaled by n^-1 2^32 for the pointwise step, and the MMGW form of 1,
+// R/2 mod N (src/transformGPU.h:743-747).  Ninv is N^-1 mod R.
+static int load_slot_consts(Ctx& c, size_t slot, const mpz_t N, const mpz_t Ninv) {
+  int rc;
+  const uint32_t* prev_ids = c.slot_ids;
+  const int prev_n = c.nslots;
+  c.slot_ids = nullptr;  // the transforms below address the slot directly
+  c.nslots = 1;
+  const size_t off = slot * c.slot_stride;
+  auto done = [&](int r) {
+    c.slot_ids = prev_ids;
+    c.nslots = prev_n;
+    return r;
+  };
+  if ((rc = upload_mpz(c, RTMP, N, slot))) return done(rc);
+  {
+    NttJob j = job(c.reg(RTMP) + off, c.reg(RNQ) + off, c.fwdQ(), c.invQ());
+    if ((rc = forward(c, &j, 1))) return done(rc);
+    if ((rc = scale_reg(c, RNQ, slot))) return done(rc);
+  }
+  if ((rc = upload_mpz(c, RTMP, Ninv, slot))) return done(rc);
+  {
+    NttJob j = job(c.reg(RTMP) + off, c.reg(RNIR) + off, c.fwdR(), c.invR());
+    if ((rc = forward(c, &j, 1))) return done(rc);
+    if ((rc = scale_reg(c, RNIR, slot))) return done(rc);
+  }
+  mpz_t R, t;
+  mpz_inits(R, t, nullptr);
+  plan_R(c.pl, R);
+  mpz_mod(t, R, N);
+  if (mpz_odd_p(t)) mpz_add(t, t, N);
+  mpz_fdiv_q_2exp(t, t, 1);
+  rc = upload_mpz(c, RONE, t, slot);
+  mpz_clears(R, t, nullptr);
+  if (rc) return done(rc);
+  HIPCHK(hipStreamSynchronize(c.stream));
+  return done(ANNIE_OK);
+}
+
+static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_ln, annie_hip_t** out) {
+  if (!out || !Nl || nl == 0) return fail(ANNIE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (a < 1) return fail(ANNIE_ERR_ARG, "base must be positive");
+  std::unique_ptr<annie_hip, void (*)(annie_hip_t*)> h(new annie_hip(), destroy);
+  Ctx& c = h->c;
+  mpz_inits(c.N, c.factor, nullptr);
+  limbs_to_mpz(c.N, Nl, nl);
+  if (mpz_even_p(c.N) || mpz_cmp_ui(c.N, 3) < 0) return fail(ANNIE_ERR_ARG, "N must be odd and >= 3");
+  c.pl = plan_for(c.N, a);
+  if (c.pl.LN < min_ln) return fail(ANNIE_ERR_SIZE, "NTT size < 2^" + std::to_string(min_ln) + " is not supported");
+  if (c.pl.LN > 23) return fail(ANNIE_ERR_SIZE, "NTT size > 2^23 is not supported");
+  if (c.pl.LN < 5) return fail(ANNIE_ERR_SIZE, "NTT size < 2^5 is not supported by this engine");
+
+  // R = B^n - 1; N must be invertible mod R (src/transformGPU.h:652-661).
+  mpz_t R, t;
+  mpz_inits(R, t, nullptr);
+  struct Clear {
+    mpz_t& a;
+    mpz_t& b;
+    ~Clear() { mpz_clears(a, b, nullptr); }
+  } clear_rt{R, t};
+  plan_R(c.pl, R);
+  if (mpz_invert(t, c.N, R) == 0) {
+    mpz_gcd(c.factor, c.N, R);
+    c.init_failed = true;
+    *out = h.release();
+    return ANNIE_INIT_FAILED;
+  }
+  int rc;
+  if ((rc = setup_device(c, device, 1))) return rc;
+  if ((rc = load_slot_consts(c, 0, c.N, t))) return rc;
   *out = h.release();
+  return ANNIE_OK;
+}
+
+// Uploads a slot list (and optionally per-slot values) through the staging
+// ring; returns device pointers valid for operations enqueued afterwards.
+static int stage_upload(Ctx& c, const uint32_t* ids, const uint32_t* vals, size_t count, const uint32_t** d_ids,
+                        const uint32_t** d_vals) {
+  Stage& st = c.stage;
+  const size_t words = count * (vals ? 2 : 1);
+  if (words > st.cap) return fail(ANNIE_ERR_ARG, "slot list longer than the batch");
+  const int e = st.pos;
+  st.pos = (st.pos + 1) % NSTAGE;
+  if (st.pending[e]) {
+    HIPCHK(hipEventSynchronize(st.ev[e]));
+    st.pending[e] = false;
+  }
+  uint32_t* h = st.host + size_t(e) * st.cap;
+  uint32_t* d = st.dev + size_t(e) * st.cap;
+  std::memcpy(h, ids, count * 4);
+  if (vals) std::memcpy(h + count, vals, count * 4);
+  HIPCHK(hipMemcpyAsync(d, h, words * 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipEventRecord(st.ev[e], c.stream));
+  st.pending[e] = true;
+  *d_ids = d;
+  if (d_vals) *d_vals = vals ? d + count : nullptr;
+  return ANNIE_OK;
+}
+
+// The slot list of one batch operation.
+struct SlotScope {
+  Ctx& c;
+  SlotScope(Ctx& cc, const uint32_t* ids, size_t count) : c(cc) {
+    c.slot_ids = ids;
+    c.nslots = int(count);
+  }
+  ~SlotScope() {
+    c.slot_ids = nullptr;
+    c.nslots = 1;
+  }
+};
+
+// Slot lists: non-empty, ids in range, each id once.
+static int check_slots(const Ctx& c, const uint32_t* ids, size_t count) {
+  if (!ids || count == 0 || count > c.nslots_alloc) return fail(ANNIE_ERR_ARG, "bad slot list");
+  std::vector<uint8_t> seen(c.nslots_alloc, 0);
+  for (size_t i = 0; i < count; ++i) {
+    if (ids[i] >= c.nslots_alloc) return fail(ANNIE_ERR_ARG, "slot id out of range");
+    if (seen[ids[i]]++) return fail(ANNIE_ERR_ARG, "slot listed twice");
+    if (!c.slotN[ids[i]]) return fail(ANNIE_ERR_ARG, "slot has no modulus loaded");
+  }
   return ANNIE_OK;
 }
 
@@ -859,6 +1019,147 @@ int annie_hip_read_tq(annie_hip_t* h, uint32_t* out, size_t cap_words) {
   if (!out || cap_words < size_t(DIGIT_LIMBS) * c.n) return fail(ANNIE_ERR_ARG, "buffer too small");
   HIPCHK(hipMemcpyAsync(out, c.tq(), size_t(DIGIT_LIMBS) * c.n * 4, hipMemcpyDeviceToHost, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
+  return ANNIE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches: many moduli of one length class (one plan) in one allocation; every
+// operation runs for a list of slots with one set of launches.
+
+int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int device, annie_hip_t** out) {
+  if (!out) return fail(ANNIE_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (K < 2 || K > KMAX || (pck != 1 && pck != 2 && pck != 4) || LN < 5 || LN > 23 || lbase < 2 || lbase > 31 ||
+      pck * lbase > 32 * DIGIT_LIMBS - 1)
+    return fail(ANNIE_ERR_ARG, "bad plan");
+  if (slots == 0 || slots > 32767) return fail(ANNIE_ERR_ARG, "slot count must be in [1, 32767]");
+  std::unique_ptr<annie_hip, void (*)(annie_hip_t*)> h(new annie_hip(), destroy);
+  Ctx& c = h->c;
+  mpz_inits(c.N, c.factor, nullptr);
+  c.pl.K = K;
+  c.pl.pck = pck;
+  c.pl.LN = LN;
+  c.pl.lbase = lbase;
+  int rc;
+  if ((rc = setup_device(c, device, slots))) return rc;
+  c.slotN.resize(slots);
+  Stage& st = c.stage;
+  st.cap = 2 * slots;
+  if (hipMalloc(&st.dev, NSTAGE * st.cap * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "staging");
+  if (hipHostMalloc(&st.host, NSTAGE * st.cap * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "pinned staging");
+  for (int e = 0; e < NSTAGE; ++e) HIPCHK(hipEventCreateWithFlags(&st.ev[e], hipEventDisableTiming));
+  *out = h.release();
+  return ANNIE_OK;
+}
+
+int annie_hip_batch_load(annie_hip_t* h, size_t slot, const uint64_t* N, size_t nlimbs, int32_t a, uint64_t* factor,
+                         size_t cap, size_t* nout) {
+  if (!h || !N || nlimbs == 0 || a < 1) return fail(ANNIE_ERR_ARG, "bad argument");
+  Ctx& c = h->c;
+  if (c.slotN.empty() || slot >= c.nslots_alloc) return fail(ANNIE_ERR_ARG, "not a batch, or slot out of range");
+  auto m = std::make_unique<Mpz>();
+  limbs_to_mpz(m->v, N, nlimbs);
+  if (mpz_even_p(m->v) || mpz_cmp_ui(m->v, 3) < 0) return fail(ANNIE_ERR_ARG, "N must be odd and >= 3");
+  const Plan pl = plan_for(m->v, a);
+  if (pl.K != c.pl.K || pl.pck != c.pl.pck || pl.LN != c.pl.LN || pl.lbase != c.pl.lbase)
+    return fail(ANNIE_ERR_SIZE, "N belongs to another length class");
+  mpz_t R, t;
+  mpz_inits(R, t, nullptr);
+  plan_R(c.pl, R);
+  if (mpz_invert(t, m->v, R) == 0) {
+    mpz_gcd(t, m->v, R);
+    const int r = mpz_to_limbs(t, factor, cap, nout);
+    mpz_clears(R, t, nullptr);
+    c.slotN[slot].reset();
+    return r == ANNIE_OK ? ANNIE_INIT_FAILED : r;
+  }
+  HIPCHK(hipSetDevice(c.device));
+  const int rc = load_slot_consts(c, slot, m->v, t);
+  mpz_clears(R, t, nullptr);
+  if (rc) return rc;
+  c.slotN[slot] = std::move(m);
+  if (nout) *nout = 0;
+  return ANNIE_OK;
+}
+
+int annie_hip_batch_set1(annie_hip_t* h, const uint32_t* slots, size_t count) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  const uint32_t* d;
+  if ((rc = stage_upload(c, slots, nullptr, count, &d, nullptr))) return rc;
+  SlotScope sc(c, d, count);
+  return copy_reg(c, R0, RONE);
+}
+
+int annie_hip_batch_copy(annie_hip_t* h, size_t dst, size_t src, const uint32_t* slots, size_t count) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  Ctx& c = h->c;
+  if (dst > 2 || src > 2) return fail(ANNIE_ERR_ARG, "register index out of range");
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  const uint32_t* d;
+  if ((rc = stage_upload(c, slots, nullptr, count, &d, nullptr))) return rc;
+  SlotScope sc(c, d, count);
+  return copy_reg(c, int(dst), int(src));
+}
+
+int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint32_t* a, size_t count) {
+  if (!h || !a) return fail(ANNIE_ERR_ARG, "null argument");
+  Ctx& c = h->c;
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  for (size_t i = 0; i < count; ++i)
+    if (a[i] < 1 || a[i] > 0x7fffffffu) return fail(ANNIE_ERR_ARG, "multiplier must be in [1, 2^31)");
+  const uint32_t *d, *da;
+  if ((rc = stage_upload(c, slots, a, count, &d, &da))) return rc;
+  SlotScope sc(c, d, count);
+  return annie::square_mul(c, 1u, da);
+}
+
+int annie_hip_batch_mul(annie_hip_t* h, size_t dst, size_t src, const uint32_t* slots, size_t count) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  Ctx& c = h->c;
+  if (dst > 2 || src > 2) return fail(ANNIE_ERR_ARG, "register index out of range");
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  const uint32_t* d;
+  if ((rc = stage_upload(c, slots, nullptr, count, &d, nullptr))) return rc;
+  SlotScope sc(c, d, count);
+  return annie::mul(c, int(dst), int(src));
+}
+
+int annie_hip_batch_get(annie_hip_t* h, size_t reg, const uint32_t* slots, size_t count, uint64_t* limbs,
+                        size_t stride, size_t* nout) {
+  if (!h || !limbs || !nout) return fail(ANNIE_ERR_ARG, "null argument");
+  Ctx& c = h->c;
+  if (reg > 2) return fail(ANNIE_ERR_ARG, "register index out of range");
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  const uint32_t* d;
+  if ((rc = stage_upload(c, slots, nullptr, count, &d, nullptr))) return rc;
+  std::vector<size_t> sl(slots, slots + count);
+  std::vector<std::unique_ptr<Mpz>> z(count);
+  std::vector<const mpz_t*> mods(count);
+  std::vector<mpz_t*> zs(count);
+  for (size_t i = 0; i < count; ++i) {
+    z[i] = std::make_unique<Mpz>();
+    zs[i] = &z[i]->v;
+    mods[i] = &c.slotN[slots[i]]->v;
+  }
+  {
+    SlotScope sc(c, d, count);
+    if ((rc = get_values(c, int(reg), sl.data(), mods.data(), zs.data(), count))) return rc;
+  }
+  for (size_t i = 0; i < count; ++i)
+    if ((rc = mpz_to_limbs(z[i]->v, limbs + i * stride, stride, nout + i))) return rc;
+  return ANNIE_OK;
+}
+
+int annie_hip_batch_sync(annie_hip_t* h) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(h->c.stream));
   return ANNIE_OK;
 }
 

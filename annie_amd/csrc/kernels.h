@@ -29,8 +29,16 @@ struct NttJob {
   const uint32_t* opnd; // pointwise operand planes (MULC / MULV), else null
 };
 
+// A launch runs its jobs for every slot of a slot list: grid index z (or y
+// for the carry) = position * njobs + job.  A slot is one test's register
+// file inside a batch allocation; the job pointers address slot 0 and slot s
+// is at + s * slot_stride words.  slot_ids == null means a single slot at
+// offset 0 (the one-test engine).
 struct NttArgs {
   NttJob job[2];
+  const uint32_t* slot_ids;
+  size_t slot_stride;
+  int njobs;
   uint32_t p[KMAX], q[KMAX];
   uint32_t sc[KMAX], scq[KMAX];  // n^-1 * 2^64 (Montgomery scale of a data product)
   int LN;        // log2 of the transform length
@@ -53,6 +61,10 @@ struct CarryJob {
 
 struct CarryArgs {
   CarryJob job[2];
+  const uint32_t* slot_ids;
+  size_t slot_stride;
+  int njobs;
+  const uint32_t* a_slot;  // per-position multiplier (replaces job.a) or null
   uint32_t p[KMAX], q[KMAX];
   uint32_t c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX];
   uint32_t c32n[KMAX], c32nq[KMAX];  // p - c32 (2^64 mod p negated) and its product with p^-1

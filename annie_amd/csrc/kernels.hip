@@ -30,6 +30,11 @@ struct Tw2 {
   uint32_t w, wq;
 };
 
+// Word offset of the slot at list position pos (0 without a slot list).
+__device__ __forceinline__ size_t slot_offset(const uint32_t* ids, size_t stride, uint32_t pos) {
+  return ids ? size_t(ids[pos]) * stride : size_t(0);
+}
+
 __device__ __forceinline__ void fwd_bfly(uint32_t& a, uint32_t& b, const Tw2 w, const uint32_t p) {
   const uint32_t t = mont_c(b, w.w, w.wq, p);
   b = submod(a, t, p);
@@ -202,13 +207,15 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr uint32_t S = 1u << M, Tl = S / NTT_R;
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
-  const NttJob& J = A.job[blockIdx.z];
+  const uint32_t nj = uint32_t(A.njobs);
+  const NttJob& J = A.job[blockIdx.z % nj];
+  const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
   const uint32_t line = threadIdx.x / Tl, tg = threadIdx.x % Tl;
   const uint32_t row = blockIdx.x * uint32_t(A.cols) + line;
-  const size_t base = size_t(k) * n + size_t(row) * S;
+  const size_t base = so + size_t(k) * n + size_t(row) * S;
   const uint32_t lineRoot = (1u << A.m1) + row;
   const TwDev* twf = J.fwd + size_t(k) * n;
   const TwDev* twi = J.inv + size_t(k) * n;
@@ -267,14 +274,16 @@ template <int M>
 __global__ void __launch_bounds__(1024) k_col_fwd(const NttArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
-  const NttJob& J = A.job[blockIdx.z];
+  const uint32_t nj = uint32_t(A.njobs);
+  const NttJob& J = A.job[blockIdx.z % nj];
+  const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   const size_t n2 = n >> M;
-  const size_t base = size_t(k) * n + size_t(col_group()) * C + line;
+  const size_t base = so + size_t(k) * n + size_t(col_group()) * C + line;
   const TwDev* twf = J.fwd + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
@@ -293,14 +302,16 @@ template <int M>
 __global__ void __launch_bounds__(1024) k_col_inv(const NttArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
-  const NttJob& J = A.job[blockIdx.z];
+  const uint32_t nj = uint32_t(A.njobs);
+  const NttJob& J = A.job[blockIdx.z % nj];
+  const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   const size_t n2 = n >> M;
-  const size_t base = size_t(k) * n + size_t(col_group()) * C + line;
+  const size_t base = so + size_t(k) * n + size_t(col_group()) * C + line;
   const TwDev* twi = J.inv + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
@@ -660,7 +671,15 @@ template <int K, int QS>
 __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   constexpr int L = K + 1;
   __shared__ uint32_t cl[L][CARRY_WG];
-  const CarryJob& J = A.job[blockIdx.y];
+  const uint32_t nj = uint32_t(A.njobs);
+  CarryJob J = A.job[blockIdx.y % nj];
+  {
+    const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.y / nj);
+    J.src += so;
+    J.dst += so;
+    if (J.sub != nullptr) J.sub += so;
+    if (A.a_slot != nullptr) J.a = A.a_slot[blockIdx.y / nj];
+  }
   const size_t n = size_t(1) << A.LN;
   const size_t nb = n / CARRY_L;
   const uint32_t t = threadIdx.x;
@@ -727,6 +746,17 @@ __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const Car
     dig[3][l] = v.w;
   }
   store_residues<K>(dig, A, dst, n, i0);
+}
+
+// Register copy for each slot of a list: words [0, len) at dst_off <- src_off
+// (offsets in words from the slot base).  Grid: (blocks, slots).
+__global__ void k_copy_slots(uint32_t* base, size_t dst_off, size_t src_off, size_t len, const uint32_t* slot_ids,
+                             size_t slot_stride) {
+  const size_t so = slot_offset(slot_ids, slot_stride, blockIdx.y);
+  const uint4* src = reinterpret_cast<const uint4*>(base + so + src_off);
+  uint4* dst = reinterpret_cast<uint4*>(base + so + dst_off);
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len / 4; i += size_t(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
 }
 
 // Explicit instantiations, looked up by the host engine: row lines of 2^5 ..

@@ -1,0 +1,570 @@
+// Batched and served tests: many Fermat / Euler (Proth) tests run together,
+// one engine batch per transform-length class, with continuous admission.
+//
+// Each test runs the protocol of annie::check (src/annie.h:293-404): set1,
+// the hot loop of square_mul(bit ? a : 1) with the Gerbicz-Li product
+// mul(Reg1, Reg0) every B iterations (src/annie.h:319-341), the final
+// d(t+1) = d(t) * result, the residue and verdict (src/annie.h:343-366) and
+// the Gerbicz-Li verification a^(res - 2^B) ... (src/annie.h:368-404).  Here
+// that sequence is a state machine per test that yields one register
+// operation at a time; every tick the driver takes one operation from each
+// test of a class and issues them grouped by kind, so one set of launches
+// squares every test of the class.  A test that finishes frees its slot and
+// the next waiting test of its class is admitted at once (continuous
+// batching); tests that arrive later (serving) wait until their arrival time.
+// Classes are keyed by the plan (K, pck, LN, lbase) the reference's size
+// selection gives for (N, a), src/transformGPU.h:798-857.
+#include <gmp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "annie_hip.h"
+#include "annie_host.h"
+#include "transform.h"
+
+namespace annie_host {
+
+namespace {
+
+struct Big {
+  mpz_t v;
+  Big() { mpz_init(v); }
+  ~Big() { mpz_clear(v); }
+  Big(const Big&) = delete;
+  Big& operator=(const Big&) = delete;
+};
+
+typedef std::tuple<int, int, int, int> PlanKey;  // K, pck, LN, lbase
+
+// A batch of slots of one plan.  Operations act on the listed slots.
+class BatchEngine {
+ public:
+  virtual ~BatchEngine() {}
+  // 0: loaded; 1: gcd(N, R) > 1, factor set (src/transformGPU.h:652-661).
+  virtual int load(size_t slot, const mpz_t N, int32_t a, mpz_t factor) = 0;
+  virtual void set1(const std::vector<uint32_t>& s) = 0;
+  virtual void copy(int dst, int src, const std::vector<uint32_t>& s) = 0;
+  virtual void square_mul(const std::vector<uint32_t>& s, const std::vector<uint32_t>& a) = 0;
+  virtual void mul(int dst, int src, const std::vector<uint32_t>& s) = 0;
+  virtual void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) = 0;
+};
+
+// The gfx950 engine's batch C ABI.
+class HipBatch : public BatchEngine {
+  annie_hip_t* h_ = nullptr;
+  size_t stride_ = 0;
+  std::vector<uint64_t> limbs_;
+  std::vector<size_t> nout_;
+
+  void check(int rc, const char* what) {
+    if (rc != ANNIE_OK) throw std::runtime_error(std::string(what) + ": " + annie_hip_last_error());
+  }
+
+ public:
+  HipBatch(const PlanKey& k, size_t slots, int device) {
+    const int K = std::get<0>(k), pck = std::get<1>(k), LN = std::get<2>(k), lbase = std::get<3>(k);
+    check(annie_hip_batch_create(K, pck, LN, lbase, slots, device, &h_), "batch create");
+    stride_ = ((size_t(pck) * size_t(lbase)) << LN) / 64 + 2;
+  }
+  ~HipBatch() override { annie_hip_destroy(h_); }
+  int load(size_t slot, const mpz_t N, int32_t a, mpz_t factor) override {
+    const size_t nl = mpz_size(N);
+    std::vector<uint64_t> l(nl);
+    mpz_export(l.data(), nullptr, -1, 8, 0, 0, N);
+    std::vector<uint64_t> f(nl + 1);
+    size_t nf = 0;
+    const int rc = annie_hip_batch_load(h_, slot, l.data(), nl, a, f.data(), f.size(), &nf);
+    if (rc == ANNIE_INIT_FAILED) {
+      mpz_import(factor, nf, -1, 8, 0, 0, f.data());
+      return 1;
+    }
+    check(rc, "batch load");
+    return 0;
+  }
+  void set1(const std::vector<uint32_t>& s) override { check(annie_hip_batch_set1(h_, s.data(), s.size()), "set1"); }
+  void copy(int dst, int src, const std::vector<uint32_t>& s) override {
+    check(annie_hip_batch_copy(h_, size_t(dst), size_t(src), s.data(), s.size()), "copy");
+  }
+  void square_mul(const std::vector<uint32_t>& s, const std::vector<uint32_t>& a) override {
+    check(annie_hip_batch_square_mul(h_, s.data(), a.data(), s.size()), "square_mul");
+  }
+  void mul(int dst, int src, const std::vector<uint32_t>& s) override {
+    check(annie_hip_batch_mul(h_, size_t(dst), size_t(src), s.data(), s.size()), "mul");
+  }
+  void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) override {
+    limbs_.assign(stride_ * s.size(), 0);
+    nout_.assign(s.size(), 0);
+    check(annie_hip_batch_get(h_, size_t(reg), s.data(), s.size(), limbs_.data(), stride_, nout_.data()), "get");
+    for (size_t i = 0; i < s.size(); ++i) mpz_import(out[i]->v, nout_[i], -1, 8, 0, 0, limbs_.data() + i * stride_);
+  }
+};
+
+// The same operations on exact GMP transforms, one per slot (CPU; used to
+// test the driver and the state machine without a GPU).
+class GmpBatch : public BatchEngine {
+  std::vector<std::unique_ptr<transform>> t_;
+
+ public:
+  explicit GmpBatch(size_t slots) : t_(slots) {}
+  int load(size_t slot, const mpz_t N, int32_t a, mpz_t factor) override {
+    t_[slot].reset(create_transform_gmp(N, a));
+    if (t_[slot]->init_failed()) {
+      t_[slot]->get_factor(factor);
+      t_[slot].reset();
+      return 1;
+    }
+    return 0;
+  }
+  void set1(const std::vector<uint32_t>& s) override {
+    for (uint32_t i : s) t_[i]->set1();
+  }
+  void copy(int dst, int src, const std::vector<uint32_t>& s) override {
+    for (uint32_t i : s) t_[i]->copy(size_t(dst), size_t(src));
+  }
+  void square_mul(const std::vector<uint32_t>& s, const std::vector<uint32_t>& a) override {
+    for (size_t j = 0; j < s.size(); ++j) t_[s[j]]->square_mul(int32_t(a[j]));
+  }
+  void mul(int dst, int src, const std::vector<uint32_t>& s) override {
+    for (uint32_t i : s) t_[i]->mul(size_t(dst), size_t(src));
+  }
+  void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) override {
+    for (size_t j = 0; j < s.size(); ++j) t_[s[j]]->get(out[j]->v, size_t(reg));
+  }
+};
+
+// Exponent block of the Gerbicz-Li proof (src/annie.h:95-101).
+int gl_block_size(size_t size) {
+  int lg = -1;
+  for (uint32_t v = uint32_t(size); v; v >>= 1) ++lg;
+  const size_t L = size_t(2) << (lg / 2);
+  return int((size - 1) / L) + 1;
+}
+
+enum OpKind { OP_SET1, OP_COPY, OP_SQR, OP_MUL, OP_GET, OP_DONE };
+struct Op {
+  OpKind kind;
+  int dst, src;
+  uint32_t a;
+};
+
+// The protocol of one test as a sequence of register operations.
+class TestMachine {
+  enum Phase {
+    SET1, COPY1, LOOP, LOOP_MUL, FIN_COPY, FIN_MUL, GET_T, EULER_SQR, GET_RES, VSET1, V1, VMUL, V2, GET_R1, GET_R2, DONE
+  };
+  Phase ph_ = SET1;
+  long i_ = 0;
+  int B_ = 1, rs_ = 0;
+  Big e_, res_, r1_;
+  const __mpz_struct* N_;
+
+ public:
+  int32_t a;
+  bool euler;
+  long iterations = 0;
+  bool is_prp = false, gl_ok = false;
+  uint64_t res64 = 0;
+
+  TestMachine(const mpz_t N, int32_t base, bool is_euler) : N_(N), a(base), euler(is_euler) {
+    mpz_sub_ui(e_.v, N, 1);
+    if (euler) mpz_tdiv_q_2exp(e_.v, e_.v, 1);
+    B_ = gl_block_size(mpz_sizeinbase(e_.v, 2));
+    i_ = long(mpz_sizeinbase(e_.v, 2)) - 1;
+  }
+  bool done() const { return ph_ == DONE; }
+
+  Op op() const {
+    switch (ph_) {
+      case SET1: return {OP_SET1, 0, 0, 0};
+      case COPY1: return {OP_COPY, 1, 0, 0};
+      case LOOP: return {OP_SQR, 0, 0, mpz_tstbit(e_.v, mp_bitcnt_t(i_)) ? uint32_t(a) : 1u};
+      case LOOP_MUL: return {OP_MUL, 1, 0, 0};
+      case FIN_COPY: return {OP_COPY, 2, 0, 0};
+      case FIN_MUL: return {OP_MUL, 2, 1, 0};
+      case GET_T: return {OP_GET, 0, 0, 0};
+      case EULER_SQR: return {OP_SQR, 0, 0, 1u};
+      case GET_RES: return {OP_GET, 0, 0, 0};
+      case VSET1: return {OP_SET1, 0, 0, 0};
+      case V1: return {OP_SQR, 0, 0, mpz_tstbit(res_.v, mp_bitcnt_t(i_)) ? uint32_t(a) : 1u};
+      case VMUL: return {OP_MUL, 0, 1, 0};
+      case V2: return {OP_SQR, 0, 0, (i_ < rs_ && mpz_tstbit(res_.v, mp_bitcnt_t(i_))) ? uint32_t(a) : 1u};
+      case GET_R1: return {OP_GET, 0, 0, 0};
+      case GET_R2: return {OP_GET, 2, 0, 0};
+      default: return {OP_DONE, 0, 0, 0};
+    }
+  }
+
+  // Advances past the operation returned by op(); v is the value of a GET.
+  void advance(const mpz_t v) {
+    switch (ph_) {
+      case SET1: ph_ = COPY1; break;
+      case COPY1: ph_ = LOOP; break;
+      case LOOP:
+        ++iterations;
+        if ((i_ % B_ == 0) && (i_ / B_ != 0)) {
+          ph_ = LOOP_MUL;
+          break;
+        }
+        // fall through
+      case LOOP_MUL:
+        if (--i_ < 0) ph_ = FIN_COPY;
+        else ph_ = LOOP;
+        break;
+      case FIN_COPY: ph_ = FIN_MUL; break;
+      case FIN_MUL: ph_ = GET_T; break;
+      case GET_T: {  // result ?= -1 (Euler) or 1 (Fermat), src/annie.h:349-356
+        Big t;
+        if (euler) {
+          mpz_add_ui(t.v, v, 1);
+          if (mpz_cmp(t.v, N_) >= 0) mpz_sub(t.v, t.v, N_);
+        } else {
+          mpz_sub_ui(t.v, v, 1);
+        }
+        is_prp = mpz_sgn(t.v) == 0;
+        ph_ = euler ? EULER_SQR : GET_RES;
+        break;
+      }
+      case EULER_SQR: ph_ = GET_RES; break;
+      case GET_RES: {
+        res64 = mpz_size(v) ? uint64_t(mpz_getlimbn(v, 0)) : 0;
+        // res = sum of the B-bit chunks of e (src/annie.h:368-375)
+        Big t;
+        mpz_set_ui(res_.v, 0);
+        while (mpz_sgn(e_.v) != 0) {
+          mpz_tdiv_r_2exp(t.v, e_.v, mp_bitcnt_t(B_));
+          mpz_add(res_.v, res_.v, t.v);
+          mpz_tdiv_q_2exp(e_.v, e_.v, mp_bitcnt_t(B_));
+        }
+        rs_ = int(mpz_sizeinbase(res_.v, 2));
+        ph_ = VSET1;
+        break;
+      }
+      case VSET1:
+        i_ = rs_ - 1;
+        ph_ = (i_ >= B_) ? V1 : VMUL;
+        break;
+      case V1:
+        if (--i_ < B_) ph_ = VMUL;
+        break;
+      case VMUL:
+        i_ = B_ - 1;
+        ph_ = V2;
+        break;
+      case V2:
+        if (--i_ < 0) ph_ = GET_R1;
+        break;
+      case GET_R1:
+        mpz_set(r1_.v, v);
+        ph_ = GET_R2;
+        break;
+      case GET_R2:
+        gl_ok = mpz_cmp(r1_.v, v) == 0;
+        ph_ = DONE;
+        break;
+      default: break;
+    }
+  }
+};
+
+std::string hex64(uint64_t u) {
+  char b[32];
+  std::snprintf(b, sizeof(b), "%016llX", (unsigned long long)u);
+  return b;
+}
+
+std::string short_name(const mpz_t n) {
+  std::string s(mpz_sizeinbase(n, 10) + 2, '\0');
+  mpz_get_str(&s[0], 10, n);
+  s = s.c_str();
+  if (s.size() <= 40) return s;
+  return s.substr(0, 8) + "[" + std::to_string(s.size() - 16) + " other digits]" + s.substr(s.size() - 8);
+}
+
+struct Job {
+  size_t idx;
+  Big N;
+  int32_t a = 2;
+  bool euler = false;
+  double arrival = 0;
+  PlanKey key;
+  std::unique_ptr<TestMachine> m;
+};
+
+struct Class {
+  PlanKey key;
+  std::unique_ptr<BatchEngine> eng;
+  size_t slots = 0;
+  std::vector<Job*> slot;      // occupant per slot (null: free)
+  std::deque<Job*> queue;      // waiting tests, in arrival order
+  size_t active = 0;
+  size_t remaining = 0;        // tests of this class not yet finished
+};
+
+}  // namespace
+
+static int batch_run(const annie_batch_test* tests, size_t count, const annie_batch_opts& o,
+                     annie_batch_result* out, annie_batch_stats* st) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
+  auto now = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+  std::memset(out, 0, count * sizeof(annie_batch_result));
+  if (st) std::memset(st, 0, sizeof(*st));
+
+  std::vector<std::unique_ptr<Job>> jobs;
+  std::map<PlanKey, Class> classes;
+  for (size_t t = 0; t < count; ++t) {
+    annie_batch_result& r = out[t];
+    r.arrival = tests[t].arrival;
+    auto j = std::make_unique<Job>();
+    j->idx = t;
+    mpz_import(j->N.v, tests[t].nlimbs, -1, 8, 0, 0, tests[t].N);
+    j->euler = tests[t].is_euler != 0;
+    j->arrival = tests[t].arrival;
+    const std::string ns = short_name(j->N.v);
+    // Decided without the loop (src/annie.h:222-247).
+    if (ns.size() <= 15) {
+      const int pp = mpz_probab_prime_p(j->N.v, 20);
+      r.status = 4;
+      r.is_prp = pp != 0;
+      std::snprintf(r.message, sizeof(r.message), "%s", (ns + (pp == 0 ? " is composite." : " is prime.")).c_str());
+      continue;
+    }
+    if (mpz_even_p(j->N.v)) {
+      r.status = 4;
+      std::snprintf(r.message, sizeof(r.message), "%s is even.", ns.c_str());
+      continue;
+    }
+    int32_t a = 2;  // src/annie.h:249-254
+    if (j->euler) {
+      while (mpz_si_kronecker(a, j->N.v) != -1) {
+        ++a;
+        const int32_t s = int32_t(std::lround(std::sqrt(double(a))));
+        if (s * s == a) ++a;
+      }
+    } else if (tests[t].a_ref != 0) {
+      a = tests[t].a_ref;
+    }
+    j->a = a;
+    r.a = a;
+    int K = 0, pck = 0, LN = 0, lbase = 0;
+    if (annie_hip_select_plan(tests[t].N, tests[t].nlimbs, a, &K, &pck, &LN, &lbase) != ANNIE_OK) {
+      r.status = -1;
+      std::snprintf(r.message, sizeof(r.message), "Error: %s.", annie_hip_last_error());
+      continue;
+    }
+    if (LN < 5 || LN > 23) {
+      r.status = -1;
+      std::snprintf(r.message, sizeof(r.message), "Error: no transform of this size.");
+      continue;
+    }
+    j->key = PlanKey(K, pck, LN, lbase);
+    Class& c = classes[j->key];
+    c.key = j->key;
+    c.remaining++;
+    jobs.push_back(std::move(j));
+  }
+  for (auto& j : jobs) classes[j->key].queue.push_back(j.get());
+  for (auto& kv : classes)
+    std::stable_sort(kv.second.queue.begin(), kv.second.queue.end(),
+                     [](const Job* x, const Job* y) { return x->arrival < y->arrival; });
+
+  // Slots per class: every test of the class, capped by max_slots and by an
+  // equal share of the HBM budget (slot = 12 registers of K n words + digit
+  // buffers; each class also holds 4 twiddle tables).
+  const double budget = o.hbm_budget_mb > 0 ? o.hbm_budget_mb * 1048576.0 : 64.0 * 1073741824.0;
+  const size_t ncls = std::max<size_t>(classes.size(), 1);
+  for (auto& kv : classes) {
+    Class& c = kv.second;
+    const double K = std::get<0>(c.key), n = double(size_t(1) << std::get<2>(c.key));
+    const double slot_bytes = (12 * K + 6) * n * 4, fixed = 16 * K * n;
+    size_t s = c.remaining;
+    if (o.max_slots > 0) s = std::min(s, o.max_slots);
+    const double share = budget / double(ncls) - fixed;
+    s = std::min(s, size_t(std::max(1.0, std::floor(share / slot_bytes))));
+    c.slots = std::max<size_t>(s, 1);
+  }
+
+  size_t done = 0;
+  const size_t total = jobs.size();
+  double squarings = 0;
+  bool stopped = false;
+  std::vector<Big*> vals;
+  std::vector<std::unique_ptr<Big>> store;
+  auto finish = [&](Class& c, size_t s) {
+    Job* j = c.slot[s];
+    annie_batch_result& r = out[j->idx];
+    TestMachine& m = *j->m;
+    r.finished = now();
+    r.iterations = m.iterations;
+    r.is_prp = m.is_prp;
+    r.res64 = m.res64;
+    r.gl_ok = m.gl_ok;
+    const std::string ns = short_name(j->N.v);
+    std::string msg;
+    if (m.gl_ok) {
+      r.status = 0;
+      msg = m.is_prp ? ns + " is a " + (m.euler ? "Euler" : "Fermat") + " probable prime, a = " + std::to_string(m.a) + "."
+                     : ns + " is composite, res64 = " + hex64(m.res64) + ", a = " + std::to_string(m.a) + ".";
+    } else {
+      r.status = 1;
+      msg = ns + ": validation failed!";
+    }
+    std::snprintf(r.message, sizeof(r.message), "%s", msg.c_str());
+    if (o.verbose) std::printf("%s\n", msg.c_str()), std::fflush(stdout);
+    c.slot[s] = nullptr;
+    c.active--;
+    c.remaining--;
+    j->m.reset();
+    ++done;
+  };
+
+  while (done < total && !stopped) {
+    const double tnow = now();
+    if (o.time_limit > 0 && tnow >= o.time_limit) break;
+    // Admission: free slots take arrived tests of their class.
+    for (auto& kv : classes) {
+      Class& c = kv.second;
+      while (!c.queue.empty() && (!o.realtime || c.queue.front()->arrival <= tnow)) {
+        if (!c.eng) {
+          if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
+          else c.eng.reset(new HipBatch(c.key, c.slots, o.device));
+          c.slot.assign(c.slots, nullptr);
+        }
+        size_t s = 0;
+        while (s < c.slots && c.slot[s]) ++s;
+        if (s == c.slots) break;
+        Job* j = c.queue.front();
+        c.queue.pop_front();
+        annie_batch_result& r = out[j->idx];
+        r.admitted = now();
+        Big f;
+        if (c.eng->load(s, j->N.v, j->a, f.v) == 1) {  // src/annie.h:269-283
+          r.status = 3;
+          r.finished = now();
+          std::string fs(mpz_sizeinbase(f.v, 10) + 2, '\0');
+          mpz_get_str(&fs[0], 10, f.v);
+          std::snprintf(r.message, sizeof(r.message), "%s divides %s.", fs.c_str(), short_name(j->N.v).c_str());
+          c.remaining--;
+          ++done;
+          continue;
+        }
+        j->m = std::make_unique<TestMachine>(j->N.v, j->a, j->euler);
+        c.slot[s] = j;
+        c.active++;
+      }
+    }
+    // One tick per class with active tests.
+    bool any = false;
+    for (auto& kv : classes) {
+      Class& c = kv.second;
+      if (c.active == 0) continue;
+      any = true;
+      std::vector<uint32_t> set1, sq, sqa;
+      std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
+      std::map<int, std::vector<uint32_t>> gt;
+      for (size_t s = 0; s < c.slots; ++s) {
+        if (!c.slot[s]) continue;
+        const Op op = c.slot[s]->m->op();
+        const uint32_t id = uint32_t(s);
+        switch (op.kind) {
+          case OP_SET1: set1.push_back(id); break;
+          case OP_COPY: cp[{op.dst, op.src}].push_back(id); break;
+          case OP_SQR:
+            sq.push_back(id);
+            sqa.push_back(op.a);
+            break;
+          case OP_MUL: ml[{op.dst, op.src}].push_back(id); break;
+          case OP_GET: gt[op.dst].push_back(id); break;
+          default: break;
+        }
+      }
+      if (!set1.empty()) c.eng->set1(set1);
+      for (auto& g : cp) c.eng->copy(g.first.first, g.first.second, g.second);
+      if (!sq.empty()) {
+        c.eng->square_mul(sq, sqa);
+        squarings += double(sq.size());
+      }
+      for (auto& g : ml) c.eng->mul(g.first.first, g.first.second, g.second);
+      std::vector<uint8_t> handled(c.slots, 0);
+      for (auto& g : gt) {
+        store.clear();
+        vals.clear();
+        for (size_t i = 0; i < g.second.size(); ++i) {
+          store.push_back(std::make_unique<Big>());
+          vals.push_back(store.back().get());
+        }
+        c.eng->get(g.first, g.second, vals);
+        for (size_t i = 0; i < g.second.size(); ++i) {
+          handled[g.second[i]] = 1;
+          c.slot[g.second[i]]->m->advance(vals[i]->v);
+          if (c.slot[g.second[i]]->m->done()) finish(c, g.second[i]);
+        }
+      }
+      Big none;
+      for (size_t s = 0; s < c.slots; ++s) {
+        if (!c.slot[s] || handled[s]) continue;
+        c.slot[s]->m->advance(none.v);
+        if (c.slot[s]->m->done()) finish(c, s);
+      }
+    }
+    if (!any) {
+      // Nothing running: wait for the next arrival.
+      double next = -1;
+      for (auto& kv : classes)
+        if (!kv.second.queue.empty()) next = (next < 0) ? kv.second.queue.front()->arrival : std::min(next, kv.second.queue.front()->arrival);
+      if (next < 0) break;
+      const double wait = next - now();
+      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+    }
+  }
+  // Tests cut off by the time limit.
+  for (auto& j : jobs) {
+    annie_batch_result& r = out[j->idx];
+    if (j->m) {
+      r.status = 2;
+      r.iterations = j->m->iterations;
+      std::snprintf(r.message, sizeof(r.message), "Test was aborted.");
+    } else if (r.status == 0 && r.finished == 0 && r.message[0] == 0) {
+      r.status = 2;
+      std::snprintf(r.message, sizeof(r.message), "Test was not started.");
+    }
+  }
+  if (st) {
+    st->seconds = now();
+    st->squarings = squarings;
+    st->classes = int(classes.size());
+    size_t slots = 0;
+    for (auto& kv : classes) slots += kv.second.slots;
+    st->slots = slots;
+  }
+  return 0;
+}
+
+}  // namespace annie_host
+
+extern "C" int annie_batch_run(const annie_batch_test* tests, size_t count, const annie_batch_opts* opts,
+                               annie_batch_result* results, annie_batch_stats* stats) {
+  if ((!tests && count) || !opts || (!results && count)) return -1;
+  try {
+    return annie_host::batch_run(tests, count, *opts, results, stats);
+  } catch (const std::exception& ex) {
+    for (size_t i = 0; i < count; ++i)
+      if (results[i].status == 0 && results[i].message[0] == 0) {
+        results[i].status = -1;
+        std::snprintf(results[i].message, sizeof(results[i].message), "Error: %s.", ex.what());
+      }
+    return -1;
+  }
+}

@@ -114,6 +114,31 @@ int annie_hip_write_register(annie_hip_t* ctx, size_t reg, const uint32_t* in, s
 int annie_hip_debug_square_stage(annie_hip_t* ctx, int32_t a, int stage);
 int annie_hip_read_tq(annie_hip_t* ctx, uint32_t* out, size_t cap_words);
 
+/* ---- Batches (no counterpart in the reference, which runs one test per
+ * process).  A batch holds `slots` moduli of one length class -- one plan
+ * (K, pck, LN, lbase) of the size selection, src/transformGPU.h:798-857 --
+ * with shared twiddles and per-slot registers, and runs each operation for a
+ * list of slots with one set of launches: the register operations of
+ * src/transform.h:18-45 (set1, copy, square_mul, mul, get), per slot.  Slot
+ * lists hold distinct slot ids that have a modulus loaded.  A batch is
+ * released with annie_hip_destroy. */
+int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int device, annie_hip_t** out);
+/* Loads modulus N (base a) into a slot: its plan must be the batch's.  Returns
+ * ANNIE_INIT_FAILED with gcd(N, R) in `factor` when N is not invertible mod R
+ * (src/transformGPU.h:652-661). */
+int annie_hip_batch_load(annie_hip_t* b, size_t slot, const uint64_t* N, size_t nlimbs, int32_t a, uint64_t* factor,
+                         size_t cap, size_t* nout);
+int annie_hip_batch_set1(annie_hip_t* b, const uint32_t* slots, size_t count);
+int annie_hip_batch_copy(annie_hip_t* b, size_t dst, size_t src, const uint32_t* slots, size_t count);
+/* r0 = a[i] * r0^2 in slot slots[i]. */
+int annie_hip_batch_square_mul(annie_hip_t* b, const uint32_t* slots, const uint32_t* a, size_t count);
+int annie_hip_batch_mul(annie_hip_t* b, size_t dst, size_t src, const uint32_t* slots, size_t count);
+/* Canonical values of register `reg` of each listed slot: slot i's limbs at
+ * limbs + i * stride (stride limbs each), nout[i] limbs used. */
+int annie_hip_batch_get(annie_hip_t* b, size_t reg, const uint32_t* slots, size_t count, uint64_t* limbs, size_t stride,
+                        size_t* nout);
+int annie_hip_batch_sync(annie_hip_t* b);
+
 /* Releases the context (the reference deletes the transform, src/annie.h:406). */
 void annie_hip_destroy(annie_hip_t* ctx);
 

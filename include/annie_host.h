@@ -52,6 +52,50 @@ int annie_checkpoint_name(const uint64_t* N, size_t nlimbs, char* buf, size_t ca
 /* CRC-32 of the checkpoint files (src/file.h:56-90): running value after buf. */
 uint32_t annie_crc32(uint32_t crc, const void* buf, size_t len);
 
+/* ---- Batched and served tests (no counterpart in the reference, which runs
+ * one test per process; these drive many tests of a length class through one
+ * engine batch with continuous admission). */
+
+typedef struct {
+  const uint64_t* N;          /* little-endian 64-bit limbs */
+  size_t nlimbs;
+  int is_euler;               /* Euler (Proth) test instead of Fermat */
+  int32_t a_ref;              /* Fermat base, 0 = default 2 */
+  double arrival;             /* seconds after the start of the run (serving) */
+} annie_batch_test;
+
+typedef struct {
+  int engine;                 /* 0: gfx950 engine; 1: exact GMP transforms on the CPU (tests) */
+  int device;                 /* GPU ordinal */
+  size_t max_slots;           /* slots per length class, 0 = one per test of the class */
+  double hbm_budget_mb;       /* device memory for all class batches, 0 = 64 GiB */
+  int realtime;               /* 1: admit a test only once its arrival time has passed */
+  double time_limit;          /* > 0: stop after that many seconds (unfinished tests: status 2) */
+  int verbose;                /* print result lines */
+} annie_batch_opts;
+
+typedef struct {
+  int status;                 /* as annie_check_result.status */
+  int is_prp;
+  uint64_t res64;
+  int gl_ok;
+  int32_t a;
+  long iterations;            /* loop iterations done */
+  double arrival, admitted, finished;  /* seconds after the start of the run */
+  char message[256];
+} annie_batch_result;
+
+typedef struct {
+  double seconds;             /* wall time of the run */
+  double squarings;           /* square_mul operations issued, summed over tests */
+  int classes;                /* length classes used */
+  size_t slots;               /* slots allocated over all classes */
+} annie_batch_stats;
+
+/* Runs every test to completion (or to time_limit); results[i] belongs to tests[i]. */
+int annie_batch_run(const annie_batch_test* tests, size_t count, const annie_batch_opts* opts,
+                    annie_batch_result* results, annie_batch_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,82 @@
+"""Batched / served test driver (annie_batch_run) on the CPU: the per-test
+protocol state machine and the continuous-admission driver run on exact GMP
+transforms and must reproduce the oracle's protocol (a restatement of
+annie::check, src/annie.h:217-430) test by test, whatever the slot count,
+arrival order or mix of classes."""
+import random
+
+import pytest
+
+import annie_amd
+import oracle
+from tests.helpers import proth
+
+
+def _mix(seed, count):
+    rng = random.Random(seed)
+    tests = []
+    for _ in range(count):
+        if rng.random() < 0.6:
+            k = rng.randrange(3, 1 << 12) | 1
+            tests.append({"N": proth(k, rng.randrange(1200, 3000)), "is_euler": True})
+        else:
+            tests.append({"N": rng.getrandbits(rng.randrange(2500, 6000)) | 1, "a": 3})
+    return tests
+
+
+def _expect(t):
+    return oracle.check(t["N"], is_euler=t.get("is_euler", False), a_ref=t.get("a", 0))
+
+
+def _assert_matches(tests, res):
+    for t, r in zip(tests, res):
+        o = _expect(t)
+        if o.get("init_failed"):
+            assert r["status"] == "factor", r
+            continue
+        assert r["status"] == "success", r
+        assert r["gl_ok"]
+        assert r["res64"] == o["res64"] and r["is_prp"] == o["is_prp"] and r["a"] == o["a"], (r, o)
+
+
+@pytest.mark.parametrize("max_slots", [0, 1, 3])
+def test_batch_matches_oracle(max_slots):
+    tests = _mix(11, 24)
+    res, st = annie_amd.batch_run(tests, engine="gmp", max_slots=max_slots)
+    assert st["rc"] == 0
+    _assert_matches(tests, res)
+    # every square_mul of every test is accounted for
+    assert st["squarings"] >= sum(r["iterations"] for r in res)
+
+
+def test_known_proth_primes_in_a_batch():
+    # 3*2^n+1 is prime for n = 2208, 2816, 3168 (OEIS A002253); 3*2^2209+1 is not.
+    tests = [{"N": proth(3, n), "is_euler": True} for n in (2208, 2816, 3168, 2209)]
+    res, _ = annie_amd.batch_run(tests, engine="gmp", max_slots=2)
+    assert [r["is_prp"] for r in res] == [True, True, True, False]
+    assert all(r["gl_ok"] for r in res)
+
+
+def test_served_arrivals_respected():
+    tests = _mix(12, 10)
+    for i, t in enumerate(tests):
+        t["arrival"] = 0.02 * i
+    res, _ = annie_amd.batch_run(tests, engine="gmp", max_slots=2, realtime=True)
+    _assert_matches(tests, res)
+    for r in res:
+        if r["status"] == "success":
+            assert r["admitted"] >= r["arrival"] - 1e-6
+            assert r["finished"] >= r["admitted"]
+
+
+def test_decided_without_the_loop():
+    res, _ = annie_amd.batch_run([{"N": 1000003}, {"N": 10 ** 20 + 2}, {"N": 999999999999989}], engine="gmp")
+    assert [r["status"] for r in res] == ["decided"] * 3
+    assert res[0]["is_prp"] and not res[1]["is_prp"] and res[2]["is_prp"]
+
+
+def test_time_limit_aborts_unfinished():
+    tests = [{"N": proth(3, 20000) + 0, "is_euler": True}]
+    res, st = annie_amd.batch_run(tests, engine="gmp", time_limit=0.2)
+    assert res[0]["status"] == "aborted"
+    assert 0 < res[0]["iterations"] < 20000
