@@ -352,7 +352,7 @@ class BatchResult(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("squarings", ctypes.c_double), ("classes", ctypes.c_int),
-                ("slots", ctypes.c_size_t)]
+                ("slots", ctypes.c_size_t), ("setup_seconds", ctypes.c_double)]
 
 
 def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
@@ -390,5 +390,6 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
                     "gl_ok": bool(r.gl_ok), "a": r.a, "iterations": r.iterations, "arrival": r.arrival,
                     "admitted": r.admitted, "finished": r.finished,
                     "message": r.message.decode(errors="replace")})
-    stats = {"rc": rc, "seconds": st.seconds, "squarings": st.squarings, "classes": st.classes, "slots": st.slots}
+    stats = {"rc": rc, "seconds": st.seconds, "squarings": st.squarings, "classes": st.classes, "slots": st.slots,
+             "setup_seconds": st.setup_seconds}
     return out, stats

@@ -58,7 +58,8 @@ enum Reg : int {
   RSR, RSQ,                // forward transforms of a runtime operand (mul)
   RNQ,                     // forward negacyclic transform of N, scaled by n^-1 2^32
   RNIR,                    // forward cyclic transform of N^-1 mod R, scaled by n^-1 2^32
-  RTMP,                    // scratch for get()
+  RTMP,                    // scratch for set-up uploads
+  RSNAP,                   // batches: copy of r0 taken by a flagged square_mul
   NREG
 };
 
@@ -283,13 +284,15 @@ static int convolve(Ctx& c, int mode, const NttJob* jobs, int nj) {
   return launch_row(c, mode, jobs, nj);
 }
 
-static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_slot = nullptr) {
+static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_slot = nullptr,
+                        const uint32_t* snap = nullptr) {
   CarryArgs A = c.cbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
   A.slot_ids = c.slot_ids;
   A.slot_stride = c.slot_stride;
   A.njobs = nj;
   A.a_slot = a_slot;
+  A.snap = snap;
   const size_t nb = c.n / CARRY_L;
   // CARRY_WG - 1 owned blocks per workgroup (thread 0 only hands over a carry).
   const dim3 grid(unsigned((nb + CARRY_WG - 2) / (CARRY_WG - 1)), nj * c.nslots), block(CARRY_WG);
@@ -320,7 +323,7 @@ static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_
 
 static CarryJob cjob(const uint32_t* src, uint32_t* dst, uint32_t a, int nega, int canon, int raw_out,
                      const uint32_t* sub = nullptr) {
-  CarryJob j;
+  CarryJob j{};
   j.src = src;
   j.dst = dst;
   j.sub = sub;
@@ -333,7 +336,8 @@ static CarryJob cjob(const uint32_t* src, uint32_t* dst, uint32_t a, int nega, i
 
 // reduceMMGW (src/transformGPU.h:523-538): with T_R in RTR and T_Q (raw digits)
 // in tq(): m' = T_R N^-1 mod R, canonical, + 2; dst = (m' + 2) N - T_Q mod Q.
-static int reduce(Ctx& c, uint32_t* dst, bool raw_out) {
+// snap (batches): per-slot flags; a flagged slot also gets the result in RSNAP.
+static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr) {
   int rc;
   NttJob j = job(c.reg(RTR), c.reg(RWR), c.fwdR(), c.invR(), c.reg(RNIR));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
@@ -342,7 +346,8 @@ static int reduce(Ctx& c, uint32_t* dst, bool raw_out) {
   j = job(c.reg(RTR), c.reg(RWQ), c.fwdQ(), c.invQ(), c.reg(RNQ));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
   cj = cjob(c.reg(RWQ), dst, 1u, 1, 0, raw_out ? 1 : 0, c.tq());
-  return launch_carry(c, &cj, 1);
+  if (snap != nullptr) cj.dst2 = c.reg(RSNAP);
+  return launch_carry(c, &cj, 1, nullptr, snap);
 }
 
 // squareMMGW (src/transformGPU.h:556-568).  a_slot: per-slot multipliers of a
@@ -353,7 +358,7 @@ static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
   if ((rc = convolve(c, ROW_SQR, j, 2))) return rc;
   const CarryJob cj[2] = {cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0), cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1)};
   if ((rc = launch_carry(c, cj, 2, a_slot))) return rc;
-  return reduce(c, c.reg(R0), false);
+  return reduce(c, c.reg(R0), false, a_slot);
 }
 
 // mulMMGW (src/transformGPU.h:540-554).
@@ -1111,7 +1116,7 @@ int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint
   int rc;
   if ((rc = check_slots(c, slots, count))) return rc;
   for (size_t i = 0; i < count; ++i)
-    if (a[i] < 1 || a[i] > 0x7fffffffu) return fail(ANNIE_ERR_ARG, "multiplier must be in [1, 2^31)");
+    if ((a[i] & ~SLOT_SNAP) < 1) return fail(ANNIE_ERR_ARG, "multiplier must be in [1, 2^31)");
   const uint32_t *d, *da;
   if ((rc = stage_upload(c, slots, a, count, &d, &da))) return rc;
   SlotScope sc(c, d, count);
@@ -1121,7 +1126,8 @@ int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint
 int annie_hip_batch_mul(annie_hip_t* h, size_t dst, size_t src, const uint32_t* slots, size_t count) {
   if (!h) return fail(ANNIE_ERR_ARG, "null context");
   Ctx& c = h->c;
-  if (dst > 2 || src > 2) return fail(ANNIE_ERR_ARG, "register index out of range");
+  if (dst > 2 || src > 3) return fail(ANNIE_ERR_ARG, "register index out of range");
+  if (src == 3) src = RSNAP;
   int rc;
   if ((rc = check_slots(c, slots, count))) return rc;
   const uint32_t* d;

@@ -57,7 +57,11 @@ struct CarryJob {
   int nega;             // 1: value mod B^n + 1 (wrap negated), 0: mod B^n - 1
   int canon;            // 1: reduce m' into [0, R) then add 2; 0 otherwise (mod Q needs no fix-up)
   int raw_out;          // 1: write raw digits instead of residues
+  uint32_t* dst2;       // second destination (residues) for flagged slots, or null
 };
+
+// Bit of a per-slot multiplier that also asks for the second destination.
+constexpr uint32_t SLOT_SNAP = 0x80000000u;
 
 struct CarryArgs {
   CarryJob job[2];
@@ -65,6 +69,7 @@ struct CarryArgs {
   size_t slot_stride;
   int njobs;
   const uint32_t* a_slot;  // per-position multiplier (replaces job.a) or null
+  const uint32_t* snap;    // per-position flags (SLOT_SNAP): also write job.dst2, or null
   uint32_t p[KMAX], q[KMAX];
   uint32_t c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX];
   uint32_t c32n[KMAX], c32nq[KMAX];  // p - c32 (2^64 mod p negated) and its product with p^-1

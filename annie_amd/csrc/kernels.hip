@@ -678,7 +678,8 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     J.src += so;
     J.dst += so;
     if (J.sub != nullptr) J.sub += so;
-    if (A.a_slot != nullptr) J.a = A.a_slot[blockIdx.y / nj];
+    if (A.a_slot != nullptr) J.a = A.a_slot[blockIdx.y / nj] & ~SLOT_SNAP;
+    if (J.dst2 != nullptr) J.dst2 = (A.snap != nullptr && (A.snap[blockIdx.y / nj] & SLOT_SNAP)) ? J.dst2 + so : nullptr;
   }
   const size_t n = size_t(1) << A.LN;
   const size_t nb = n / CARRY_L;
@@ -726,6 +727,7 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     }
   } else {
     store_residues<K>(dig, A, J.dst, n, i0);
+    if (J.dst2 != nullptr) store_residues<K>(dig, A, J.dst2, n, i0);
   }
 }
 

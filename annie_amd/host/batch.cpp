@@ -133,10 +133,16 @@ class GmpBatch : public BatchEngine {
   void copy(int dst, int src, const std::vector<uint32_t>& s) override {
     for (uint32_t i : s) t_[i]->copy(size_t(dst), size_t(src));
   }
+  // A snapshot square (bit 31 of a) takes the Gerbicz-Li product at once; the
+  // later product with the snapshot register is then a no-op (same result).
   void square_mul(const std::vector<uint32_t>& s, const std::vector<uint32_t>& a) override {
-    for (size_t j = 0; j < s.size(); ++j) t_[s[j]]->square_mul(int32_t(a[j]));
+    for (size_t j = 0; j < s.size(); ++j) {
+      t_[s[j]]->square_mul(int32_t(a[j] & 0x7fffffffu));
+      if (a[j] & 0x80000000u) t_[s[j]]->mul(1, 0);
+    }
   }
   void mul(int dst, int src, const std::vector<uint32_t>& s) override {
+    if (src == 3) return;
     for (uint32_t i : s) t_[i]->mul(size_t(dst), size_t(src));
   }
   void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) override {
@@ -159,16 +165,27 @@ struct Op {
   uint32_t a;
 };
 
-// The protocol of one test as a sequence of register operations.
+// Multiplier bit asking the engine to keep a copy of the new r0 (the snapshot
+// register, 3) for a Gerbicz-Li product taken later.
+constexpr uint32_t SNAP = 0x80000000u;
+
+// The protocol of one test as a sequence of register operations.  The
+// Gerbicz-Li product d(t) *= r0 of a loop step (src/annie.h:338) is deferred:
+// that step's square_mul also writes r0 to the snapshot register, and
+// mul(1, 3) follows at a tick where the driver flushes pending products for
+// all tests together, or before this test's next snapshot or the end of its
+// loop.  The products commute, so d(t) is unchanged.
 class TestMachine {
-  enum Phase {
-    SET1, COPY1, LOOP, LOOP_MUL, FIN_COPY, FIN_MUL, GET_T, EULER_SQR, GET_RES, VSET1, V1, VMUL, V2, GET_R1, GET_R2, DONE
-  };
+  enum Phase { SET1, COPY1, LOOP, FIN_COPY, FIN_MUL, GET_T, EULER_SQR, GET_RES, VSET1, V1, VMUL, V2, GET_R1, GET_R2, DONE };
   Phase ph_ = SET1;
   long i_ = 0;
   int B_ = 1, rs_ = 0;
+  bool pending_ = false;
+  Op last_{OP_DONE, 0, 0, 0};
   Big e_, res_, r1_;
   const __mpz_struct* N_;
+
+  bool boundary() const { return (i_ % B_ == 0) && (i_ / B_ != 0); }
 
  public:
   int32_t a;
@@ -184,14 +201,24 @@ class TestMachine {
     i_ = long(mpz_sizeinbase(e_.v, 2)) - 1;
   }
   bool done() const { return ph_ == DONE; }
+  bool pending() const { return pending_; }
 
-  Op op() const {
+  // Next operation; `flush`: take a pending Gerbicz-Li product now.
+  Op op(bool flush) {
+    last_ = next(flush);
+    return last_;
+  }
+
+  Op next(bool flush) const {
     switch (ph_) {
       case SET1: return {OP_SET1, 0, 0, 0};
       case COPY1: return {OP_COPY, 1, 0, 0};
-      case LOOP: return {OP_SQR, 0, 0, mpz_tstbit(e_.v, mp_bitcnt_t(i_)) ? uint32_t(a) : 1u};
-      case LOOP_MUL: return {OP_MUL, 1, 0, 0};
-      case FIN_COPY: return {OP_COPY, 2, 0, 0};
+      case LOOP:
+        if (pending_ && (flush || boundary())) return {OP_MUL, 1, 3, 0};
+        return {OP_SQR, 0, 0, (mpz_tstbit(e_.v, mp_bitcnt_t(i_)) ? uint32_t(a) : 1u) | (boundary() ? SNAP : 0u)};
+      case FIN_COPY:
+        if (pending_) return {OP_MUL, 1, 3, 0};
+        return {OP_COPY, 2, 0, 0};
       case FIN_MUL: return {OP_MUL, 2, 1, 0};
       case GET_T: return {OP_GET, 0, 0, 0};
       case EULER_SQR: return {OP_SQR, 0, 0, 1u};
@@ -206,21 +233,19 @@ class TestMachine {
     }
   }
 
-  // Advances past the operation returned by op(); v is the value of a GET.
+  // Advances past the operation last returned by op(); v is the value of a GET.
   void advance(const mpz_t v) {
+    if (last_.kind == OP_MUL && last_.src == 3) {  // deferred Gerbicz-Li product taken
+      pending_ = false;
+      return;
+    }
     switch (ph_) {
       case SET1: ph_ = COPY1; break;
       case COPY1: ph_ = LOOP; break;
       case LOOP:
         ++iterations;
-        if ((i_ % B_ == 0) && (i_ / B_ != 0)) {
-          ph_ = LOOP_MUL;
-          break;
-        }
-        // fall through
-      case LOOP_MUL:
+        if (last_.a & SNAP) pending_ = true;
         if (--i_ < 0) ph_ = FIN_COPY;
-        else ph_ = LOOP;
         break;
       case FIN_COPY: ph_ = FIN_MUL; break;
       case FIN_MUL: ph_ = GET_T; break;
@@ -310,7 +335,10 @@ struct Class {
   std::deque<Job*> queue;      // waiting tests, in arrival order
   size_t active = 0;
   size_t remaining = 0;        // tests of this class not yet finished
+  uint64_t ticks = 0;
 };
+
+constexpr uint64_t FLUSH = 16;
 
 }  // namespace
 
@@ -398,7 +426,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
 
   size_t done = 0;
   const size_t total = jobs.size();
-  double squarings = 0;
+  double squarings = 0, setup = 0;
   bool stopped = false;
   std::vector<Big*> vals;
   std::vector<std::unique_ptr<Big>> store;
@@ -422,7 +450,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       msg = ns + ": validation failed!";
     }
     std::snprintf(r.message, sizeof(r.message), "%s", msg.c_str());
-    if (o.verbose) std::printf("%s\n", msg.c_str()), std::fflush(stdout);
+    if (o.verbose) std::fprintf(stderr, "[%.3f s] %s\n", r.finished, msg.c_str());
     c.slot[s] = nullptr;
     c.active--;
     c.remaining--;
@@ -432,11 +460,12 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
 
   while (done < total && !stopped) {
     const double tnow = now();
-    if (o.time_limit > 0 && tnow >= o.time_limit) break;
+    if (o.time_limit > 0 && tnow - setup >= o.time_limit) break;
     // Admission: free slots take arrived tests of their class.
     for (auto& kv : classes) {
       Class& c = kv.second;
       while (!c.queue.empty() && (!o.realtime || c.queue.front()->arrival <= tnow)) {
+        const double ts = now();
         if (!c.eng) {
           if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
           else c.eng.reset(new HipBatch(c.key, c.slots, o.device));
@@ -450,7 +479,9 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
         annie_batch_result& r = out[j->idx];
         r.admitted = now();
         Big f;
-        if (c.eng->load(s, j->N.v, j->a, f.v) == 1) {  // src/annie.h:269-283
+        const int lr = c.eng->load(s, j->N.v, j->a, f.v);
+        setup += now() - ts;
+        if (lr == 1) {  // src/annie.h:269-283
           r.status = 3;
           r.finished = now();
           std::string fs(mpz_sizeinbase(f.v, 10) + 2, '\0');
@@ -474,9 +505,12 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       std::vector<uint32_t> set1, sq, sqa;
       std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
       std::map<int, std::vector<uint32_t>> gt;
+      // Deferred Gerbicz-Li products of all tests are taken together every
+      // FLUSH ticks (a test forces its own earlier if it must).
+      const bool flush = (++c.ticks % FLUSH) == 0;
       for (size_t s = 0; s < c.slots; ++s) {
         if (!c.slot[s]) continue;
-        const Op op = c.slot[s]->m->op();
+        const Op op = c.slot[s]->m->op(flush);
         const uint32_t id = uint32_t(s);
         switch (op.kind) {
           case OP_SET1: set1.push_back(id); break;
@@ -548,6 +582,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
     size_t slots = 0;
     for (auto& kv : classes) slots += kv.second.slots;
     st->slots = slots;
+    st->setup_seconds = setup;
   }
   return 0;
 }

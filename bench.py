@@ -125,6 +125,195 @@ def cpu_baseline_worker(skip, seconds):
                       "init_s": init_s, "kind": kind, "transform": info}), flush=True)
 
 
+def jacobi(a, n):
+    """Jacobi symbol (a/n), n odd positive."""
+    a %= n
+    r = 1
+    while a:
+        while a % 2 == 0:
+            a //= 2
+            if n % 8 in (3, 5):
+                r = -r
+        a, n = n, a
+        if a % 4 == 3 and n % 4 == 3:
+            r = -r
+        a %= n
+    return r if n == 1 else 0
+
+
+def euler_base(N):
+    """Base of the Euler (Proth) test: smallest a >= 2 with (a/N) = -1,
+    skipping perfect squares (src/annie.h:249-253)."""
+    a = 2
+    while jacobi(a, N) != -1:
+        a += 1
+        s = math.isqrt(a)
+        if s * s == a:
+            a += 1
+    return a
+
+
+def gl_block(size):
+    """Gerbicz-Li block B of an exponent of `size` bits (src/annie.h:95-101)."""
+    lg = size.bit_length() - 1
+    return (size - 1) // (2 << (lg // 2)) + 1
+
+
+def protocol_squarings(N, is_euler):
+    """square_mul operations of one whole test: the loop over e, the Euler
+    residue step and the Gerbicz-Li verification (src/annie.h:319-404)."""
+    e = (N - 1) >> 1 if is_euler else N - 1
+    L = e.bit_length()
+    B = gl_block(L)
+    res = 0
+    while e:
+        res += e & ((1 << B) - 1)
+        e >>= B
+    return L + res.bit_length() + (1 if is_euler else 0)
+
+
+def proth_batch_workload(count=4096, seed=4):
+    """4,096 Proth candidates k*2^n+1: k odd uniform in [3, 2^20), n
+    log-uniform in [2*10^4, 4*10^5] (seeded)."""
+    rng = random.Random(seed)
+    tests = []
+    for _ in range(count):
+        k = rng.randrange(3, 1 << 20) | 1
+        n = int(math.exp(math.log(2e4) + rng.random() * math.log(20)))
+        tests.append({"N": k * (1 << n) + 1, "is_euler": True, "k": k, "n": n})
+    return tests
+
+
+def bench_proth_batch(args):
+    """Batched vs plain throughput on the 4,096-test Proth workload.  Per
+    length class: the batched driver's square_mul rate over `--class-seconds`
+    (all tests of the class resident, continuous admission; wall time minus
+    batch set-up), and the plain engine's rate on one test of the class (one
+    context, one test at a time).  Tests/hour = tests / sum over classes of
+    (that class's total square_mul count / rate)."""
+    import annie_amd
+    tests = proth_batch_workload(args.tests)
+    classes = {}
+    for t in tests:
+        t["a"] = euler_base(t["N"])
+        key = annie_amd.select_plan(t["N"], t["a"])
+        classes.setdefault(key, []).append(t)
+    progress(f"{len(tests)} Proth tests in {len(classes)} length classes")
+    rows = []
+    hours_b = hours_p = 0.0
+    for key in sorted(classes, key=lambda k: (k[2], k[0])):
+        ct = classes[key]
+        work = sum(protocol_squarings(t["N"], True) for t in ct)
+        res, st = annie_amd.batch_run([{"N": t["N"], "is_euler": True} for t in ct], engine="gpu",
+                                      time_limit=args.class_seconds, max_slots=args.max_slots)
+        busy = st["seconds"] - st["setup_seconds"]
+        rate_b = st["squarings"] / busy if busy > 0 else float("nan")
+        bad = [r for r in res if r["status"] not in ("success", "aborted", "factor")]
+        if bad:
+            raise RuntimeError(f"class {key}: {bad[0]['message']}")
+        for t0 in ct:  # the first test of the class whose transform initialises (gcd(N, R) = 1)
+            T = annie_amd.Transform(t0["N"], t0["a"], min_ln=5)
+            if not T.init_failed():
+                break
+            T.close()
+        else:
+            progress(f"class {key}: no test initialises; skipped")
+            continue
+        e = (t0["N"] - 1) >> 1
+        nb = min(args.plain_steps, e.bit_length())
+        bits = [int(c) for c in format(e >> (e.bit_length() - nb), "b")]
+        T.set1()
+        T.square_mul_bits(bits[:20], t0["a"])
+        ms = T.time_square_mul_bits(bits[20:], t0["a"])
+        T.close()
+        rate_p = (len(bits) - 20) / (ms * 1e-3)
+        hours_b += work / rate_b / 3600
+        hours_p += work / rate_p / 3600
+        done = sum(r["status"] == "success" for r in res)
+        rows.append({"K": key[0], "pck": key[1], "LN": key[2], "lbase": key[3], "tests": len(ct),
+                     "square_muls": work, "batched_sq_per_s": round(rate_b, 1), "plain_sq_per_s": round(rate_p, 1),
+                     "speedup": round(rate_b / rate_p, 2), "slots": st["slots"], "finished_in_window": done})
+        progress(f"class K={key[0]} LN={key[2]} lbase={key[3]}: {len(ct)} tests, batched {rate_b:.0f} sq/s "
+                 f"({st['slots']} slots, {done} finished), plain {rate_p:.0f} sq/s")
+    tph_b = len(tests) / hours_b
+    tph_p = len(tests) / hours_p
+    return {
+        "metric": "batched Proth tests/hour (4,096 k*2^n+1, n log-uniform in [2e4, 4e5])",
+        "value": round(tph_b, 1), "unit": "tests/hour", "n_gpus": 1, "higher_is_better": True,
+        "plain_tests_per_hour": round(tph_p, 1), "speedup_vs_plain": round(tph_b / tph_p, 2),
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "4,096 Proth k*2^n+1, k odd in [3, 2^20), n log-uniform [2e4, 4e5], seed 4, Euler test",
+                   "class_seconds": args.class_seconds, "max_slots": args.max_slots},
+        "classes": rows,
+    }
+
+
+def serve_request(rng, lo=1e4, hi=4e4):
+    """One request of the serving mix: Fermat (random odd b-bit N, base 3) or
+    Proth (k odd < 2^32, n = b - bits(k), Euler test) with probability 1/2,
+    b log-uniform in [lo, hi] bits."""
+    b = int(math.exp(math.log(lo) + rng.random() * math.log(hi / lo)))
+    if rng.random() < 0.5:
+        return {"N": rng.getrandbits(b) | (1 << (b - 1)) | 1, "a": 3}
+    k = rng.randrange(3, 1 << 32) | 1
+    return {"N": k * (1 << (b - k.bit_length())) + 1, "is_euler": True}
+
+
+def serve_workload(rate, seconds, seed=5):
+    """Poisson arrivals at `rate` requests/s over `seconds` (seeded)."""
+    rng = random.Random(seed)
+    t, tests = 0.0, []
+    while True:
+        t += rng.expovariate(rate)
+        if t > seconds:
+            return tests
+        r = serve_request(rng)
+        r["arrival"] = t
+        tests.append(r)
+
+
+def bench_serve(args):
+    """Serving: a calibration run measures the batched square_mul capacity on
+    a sample of the request mix; requests then arrive (Poisson) at 70% of the
+    implied test rate for `--serve-seconds`, are admitted as they arrive into
+    free slots of their length class (continuous batching) and run to
+    completion; time-to-result = finish - arrival."""
+    import annie_amd
+    rng = random.Random(50)
+    sample = [serve_request(rng) for _ in range(200)]
+    _, st = annie_amd.batch_run(sample, engine="gpu", time_limit=args.calib_seconds, max_slots=args.max_slots)
+    cap_sq = st["squarings"] / max(st["seconds"] - st["setup_seconds"], 1e-9)
+    mean_sq = sum(protocol_squarings(t["N"], t.get("is_euler", False)) for t in sample) / len(sample)
+    rate = 0.7 * cap_sq / mean_sq
+    tests = serve_workload(rate, args.serve_seconds)
+    progress(f"serving: capacity {cap_sq:.0f} square_mul/s, mean {mean_sq:.0f} per request -> "
+             f"{rate:.2f} tests/s offered, {len(tests)} requests over {args.serve_seconds} s")
+    res, st2 = annie_amd.batch_run(tests, engine="gpu", realtime=True, max_slots=args.max_slots, verbose=True)
+    ok = [r for r in res if r["status"] in ("success", "factor")]
+    bad = [r for r in res if r["status"] not in ("success", "factor")]
+    tested = [r for r in ok if r["status"] == "success"]
+    lat = sorted(r["finished"] - r["arrival"] for r in ok)
+    lat_t = sorted(r["finished"] - r["arrival"] for r in tested)
+
+    def q(v, p):
+        return round(v[min(len(v) - 1, int(p * len(v)))], 3) if v else None
+    span = max(r["finished"] for r in ok) if ok else float("nan")
+    return {
+        "metric": "served tests/hour with p50/p99 time-to-result (mixed Fermat/Proth, 1e4-4e4 bits)",
+        "latency_note": "p50/p99/max over requests that ran the test; *_all_s also count requests answered at "
+                        "admission with a factor of R (gcd(N, R) > 1)",
+        "value": round(len(ok) / span * 3600, 1), "unit": "tests/hour", "n_gpus": 1, "higher_is_better": True,
+        "p50_s": q(lat_t, 0.50), "p99_s": q(lat_t, 0.99), "max_s": q(lat_t, 1.0),
+        "p50_all_s": q(lat, 0.50), "p99_all_s": q(lat, 0.99),
+        "offered_tests_per_s": round(rate, 3), "requests": len(tests), "completed": len(ok), "tested": len(tested),
+        "factor_found": len(ok) - len(tested), "errors": len(bad),
+        "capacity_square_mul_per_s": round(cap_sq, 1), "classes": st2["classes"], "slots": st2["slots"],
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "Poisson arrivals, 70% of measured capacity, Fermat base 3 or Proth (Euler) 1/2 each, "
+                               "b log-uniform in [1e4, 4e4] bits, seed 5", "serve_seconds": args.serve_seconds},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +323,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", type=int, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--workload", choices=["squaring", "proth-batch", "serve"], default="squaring",
+                    help="squaring: the ~40M-digit square_mul loop (default); proth-batch: 4,096 Proth tests, "
+                         "batched vs plain; serve: Poisson arrivals with continuous batching")
+    ap.add_argument("--tests", type=int, default=4096)
+    ap.add_argument("--class-seconds", type=float, default=3.0)
+    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--max-slots", type=int, default=0)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--calib-seconds", type=float, default=5.0)
     args = ap.parse_args()
 
     if args.cpu_baseline_worker is not None:
@@ -145,6 +343,11 @@ def main():
               f"multi-GPU mode (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
+    if args.workload != "squaring":
+        line = bench_proth_batch(args) if args.workload == "proth-batch" else bench_serve(args)
+        if rank == 0:
+            print(json.dumps(line), flush=True)
+        return
 
     import annie_amd
 

@@ -130,8 +130,13 @@ int annie_hip_batch_load(annie_hip_t* b, size_t slot, const uint64_t* N, size_t 
                          size_t cap, size_t* nout);
 int annie_hip_batch_set1(annie_hip_t* b, const uint32_t* slots, size_t count);
 int annie_hip_batch_copy(annie_hip_t* b, size_t dst, size_t src, const uint32_t* slots, size_t count);
-/* r0 = a[i] * r0^2 in slot slots[i]. */
+/* r0 = a[i] * r0^2 in slot slots[i], a[i] & 0x7fffffff in [1, 2^31).  When bit
+ * 31 of a[i] is set the new r0 is also written to the slot's snapshot
+ * register (register 3 of a batch), so that the Gerbicz-Li product
+ * r1 *= r0 of that step (src/annie.h:338) can be taken later, together with
+ * other slots'. */
 int annie_hip_batch_square_mul(annie_hip_t* b, const uint32_t* slots, const uint32_t* a, size_t count);
+/* r_dst *= r_src; src may be 3, the snapshot register. */
 int annie_hip_batch_mul(annie_hip_t* b, size_t dst, size_t src, const uint32_t* slots, size_t count);
 /* Canonical values of register `reg` of each listed slot: slot i's limbs at
  * limbs + i * stride (stride limbs each), nout[i] limbs used. */

@@ -70,8 +70,9 @@ typedef struct {
   size_t max_slots;           /* slots per length class, 0 = one per test of the class */
   double hbm_budget_mb;       /* device memory for all class batches, 0 = 64 GiB */
   int realtime;               /* 1: admit a test only once its arrival time has passed */
-  double time_limit;          /* > 0: stop after that many seconds (unfinished tests: status 2) */
-  int verbose;                /* print result lines */
+  double time_limit;          /* > 0: stop after that many seconds, not counting batch set-up
+                                 (unfinished tests: status 2) */
+  int verbose;                /* print each result line on stderr as it is decided */
 } annie_batch_opts;
 
 typedef struct {
@@ -90,6 +91,7 @@ typedef struct {
   double squarings;           /* square_mul operations issued, summed over tests */
   int classes;                /* length classes used */
   size_t slots;               /* slots allocated over all classes */
+  double setup_seconds;       /* part of `seconds` spent creating batches and loading moduli */
 } annie_batch_stats;
 
 /* Runs every test to completion (or to time_limit); results[i] belongs to tests[i]. */
