@@ -90,6 +90,7 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_batch_mul.argtypes = [h, sz, sz, u32p, sz]
     lib.annie_hip_batch_get.argtypes = [h, sz, u32p, sz, u64p, sz, ctypes.POINTER(sz)]
     lib.annie_hip_batch_sync.argtypes = [h]
+    lib.annie_hip_batch_set_graphs.argtypes = [h, ctypes.c_int]
     lib.annie_hip_destroy.argtypes = [h]
     lib.annie_hip_destroy.restype = None
     lib.annie_hip_last_error.restype = ctypes.c_char_p
@@ -340,7 +341,7 @@ class BatchTest(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int), ("device", ctypes.c_int), ("max_slots", ctypes.c_size_t),
                 ("hbm_budget_mb", ctypes.c_double), ("realtime", ctypes.c_int), ("time_limit", ctypes.c_double),
-                ("verbose", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("graphs", ctypes.c_int), ("flush_ticks", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -356,7 +357,7 @@ class BatchStats(ctypes.Structure):
 
 
 def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
-              verbose=False):
+              verbose=False, graphs=False, flush_ticks=0):
     """Run many tests together, one engine batch per length class, with
     continuous admission (annie_batch_run in include/annie_host.h).
 
@@ -379,7 +380,7 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
         keep.append(limbs)
         arr[i] = BatchTest(limbs, n, 1 if t.get("is_euler") else 0, int(t.get("a", 0)), float(t.get("arrival", 0.0)))
     opts = BatchOpts(0 if engine == "gpu" else 1, device, max_slots, hbm_budget_mb, 1 if realtime else 0,
-                     time_limit, 1 if verbose else 0)
+                     time_limit, 1 if verbose else 0, 1 if graphs else 0, int(flush_ticks))
     res = (BatchResult * max(len(tests), 1))()
     st = BatchStats()
     rc = lib.annie_batch_run(arr, len(tests), ctypes.byref(opts), res, ctypes.byref(st))

@@ -101,6 +101,18 @@ struct Ctx {
   int nslots = 1;
   Stage stage;
   std::vector<std::unique_ptr<Mpz>> slotN;  // modulus of each slot (batches)
+  // Batched squaring replayed from captured HIP graphs: the slot list and
+  // multipliers live at fixed device addresses (graph_buf: ids, then
+  // multipliers), one executable graph per list length (grid sizes).
+  bool use_graphs = false;
+  uint32_t* graph_buf = nullptr;
+  struct Graph {
+    size_t count;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
+  std::vector<Graph> graphs;
+  uint64_t graph_clock = 0;
   TwDev* tw = nullptr;        // 4 tables * K * n roots (Montgomery form)
   mpz_t N, factor;
   bool init_failed = false;
@@ -540,6 +552,8 @@ static void destroy(annie_hip_t* h) {
   }
   if (c.mem) (void)hipFree(c.mem);
   if (c.tw) (void)hipFree(c.tw);
+  for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
+  if (c.graph_buf) (void)hipFree(c.graph_buf);
   if (c.stage.dev) (void)hipFree(c.stage.dev);
   if (c.stage.host) (void)hipHostFree(c.stage.host);
   for (hipEvent_t e : c.stage.ev)
@@ -691,7 +705,7 @@ static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_
 // Uploads a slot list (and optionally per-slot values) through the staging
 // ring; returns device pointers valid for operations enqueued afterwards.
 static int stage_upload(Ctx& c, const uint32_t* ids, const uint32_t* vals, size_t count, const uint32_t** d_ids,
-                        const uint32_t** d_vals) {
+                        const uint32_t** d_vals, uint32_t* dst = nullptr) {
   Stage& st = c.stage;
   const size_t words = count * (vals ? 2 : 1);
   if (words > st.cap) return fail(ANNIE_ERR_ARG, "slot list longer than the batch");
@@ -702,7 +716,7 @@ static int stage_upload(Ctx& c, const uint32_t* ids, const uint32_t* vals, size_
     st.pending[e] = false;
   }
   uint32_t* h = st.host + size_t(e) * st.cap;
-  uint32_t* d = st.dev + size_t(e) * st.cap;
+  uint32_t* d = dst ? dst : st.dev + size_t(e) * st.cap;
   std::memcpy(h, ids, count * 4);
   if (vals) std::memcpy(h + count, vals, count * 4);
   HIPCHK(hipMemcpyAsync(d, h, words * 4, hipMemcpyHostToDevice, c.stream));
@@ -1110,6 +1124,45 @@ int annie_hip_batch_copy(annie_hip_t* h, size_t dst, size_t src, const uint32_t*
   return copy_reg(c, int(dst), int(src));
 }
 
+// Executable graph of one batched squaring over `count` slots whose ids and
+// multipliers are read from graph_buf (captured on first use).
+static int square_graph(Ctx& c, size_t count, hipGraphExec_t* out) {
+  for (auto& g : c.graphs)
+    if (g.count == count) {
+      g.used = ++c.graph_clock;
+      *out = g.exec;
+      return ANNIE_OK;
+    }
+  constexpr size_t MAX_GRAPHS = 8;
+  if (c.graphs.size() >= MAX_GRAPHS) {
+    auto lru = std::min_element(c.graphs.begin(), c.graphs.end(),
+                                [](const Ctx::Graph& x, const Ctx::Graph& y) { return x.used < y.used; });
+    HIPCHK(hipStreamSynchronize(c.stream));
+    (void)hipGraphExecDestroy(lru->exec);
+    c.graphs.erase(lru);
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+  int rc;
+  {
+    SlotScope sc(c, c.graph_buf, count);
+    rc = annie::square_mul(c, 1u, c.graph_buf + count);
+  }
+  const hipError_t e = hipStreamEndCapture(c.stream, &graph);
+  if (rc != ANNIE_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ANNIE_ERR, std::string("graph capture: ") + hipGetErrorString(e));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(ANNIE_ERR, std::string("graph instantiate: ") + hipGetErrorString(ei));
+  c.graphs.push_back({count, exec, ++c.graph_clock});
+  *out = exec;
+  return ANNIE_OK;
+}
+
 int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint32_t* a, size_t count) {
   if (!h || !a) return fail(ANNIE_ERR_ARG, "null argument");
   Ctx& c = h->c;
@@ -1118,9 +1171,31 @@ int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint
   for (size_t i = 0; i < count; ++i)
     if ((a[i] & ~SLOT_SNAP) < 1) return fail(ANNIE_ERR_ARG, "multiplier must be in [1, 2^31)");
   const uint32_t *d, *da;
-  if ((rc = stage_upload(c, slots, a, count, &d, &da))) return rc;
-  SlotScope sc(c, d, count);
-  return annie::square_mul(c, 1u, da);
+  if (!c.use_graphs) {
+    if ((rc = stage_upload(c, slots, a, count, &d, &da))) return rc;
+    SlotScope sc(c, d, count);
+    return annie::square_mul(c, 1u, da);
+  }
+  // Graph path: this step's ids and multipliers into the fixed buffer (ids at
+  // [0, count), multipliers at [count, 2 count)), then one replay; the copy
+  // and the replay are ordered on the stream after the previous replay.
+  hipGraphExec_t exec;
+  if ((rc = square_graph(c, count, &exec))) return rc;
+  if ((rc = stage_upload(c, slots, a, count, &d, &da, c.graph_buf))) return rc;
+  HIPCHK(hipGraphLaunch(exec, c.stream));
+  return ANNIE_OK;
+}
+
+int annie_hip_batch_set_graphs(annie_hip_t* h, int on) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  Ctx& c = h->c;
+  if (c.slotN.empty()) return fail(ANNIE_ERR_ARG, "not a batch");
+  if (on && !c.graph_buf) {
+    if (hipMalloc(&c.graph_buf, 2 * c.nslots_alloc * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "graph buffers");
+  }
+  HIPCHK(hipStreamSynchronize(c.stream));
+  c.use_graphs = on != 0;
+  return ANNIE_OK;
 }
 
 int annie_hip_batch_mul(annie_hip_t* h, size_t dst, size_t src, const uint32_t* slots, size_t count) {

@@ -73,9 +73,10 @@ class HipBatch : public BatchEngine {
   }
 
  public:
-  HipBatch(const PlanKey& k, size_t slots, int device) {
+  HipBatch(const PlanKey& k, size_t slots, int device, bool graphs) {
     const int K = std::get<0>(k), pck = std::get<1>(k), LN = std::get<2>(k), lbase = std::get<3>(k);
     check(annie_hip_batch_create(K, pck, LN, lbase, slots, device, &h_), "batch create");
+    if (graphs) check(annie_hip_batch_set_graphs(h_, 1), "graphs");
     stride_ = ((size_t(pck) * size_t(lbase)) << LN) / 64 + 2;
   }
   ~HipBatch() override { annie_hip_destroy(h_); }
@@ -427,6 +428,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
   size_t done = 0;
   const size_t total = jobs.size();
   double squarings = 0, setup = 0;
+  const uint64_t flush_ticks = o.flush_ticks > 0 ? uint64_t(o.flush_ticks) : FLUSH;
   bool stopped = false;
   std::vector<Big*> vals;
   std::vector<std::unique_ptr<Big>> store;
@@ -468,7 +470,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
         const double ts = now();
         if (!c.eng) {
           if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
-          else c.eng.reset(new HipBatch(c.key, c.slots, o.device));
+          else c.eng.reset(new HipBatch(c.key, c.slots, o.device, o.graphs != 0));
           c.slot.assign(c.slots, nullptr);
         }
         size_t s = 0;
@@ -507,7 +509,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       std::map<int, std::vector<uint32_t>> gt;
       // Deferred Gerbicz-Li products of all tests are taken together every
       // FLUSH ticks (a test forces its own earlier if it must).
-      const bool flush = (++c.ticks % FLUSH) == 0;
+      const bool flush = (++c.ticks % flush_ticks) == 0;
       for (size_t s = 0; s < c.slots; ++s) {
         if (!c.slot[s]) continue;
         const Op op = c.slot[s]->m->op(flush);

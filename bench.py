@@ -204,8 +204,14 @@ def bench_proth_batch(args):
     for key in sorted(classes, key=lambda k: (k[2], k[0])):
         ct = classes[key]
         work = sum(protocol_squarings(t["N"], True) for t in ct)
-        res, st = annie_amd.batch_run([{"N": t["N"], "is_euler": True} for t in ct], engine="gpu",
-                                      time_limit=args.class_seconds, max_slots=args.max_slots)
+        batch = [{"N": t["N"], "is_euler": True} for t in ct]
+        cfg = {"max_slots": args.max_slots}
+        if args.autotune:
+            from annie_amd import autotune
+            cfg = autotune.tuned(key, batch, path=args.tune_cache, seconds=args.tune_seconds,
+                                 configs=[{"graphs": g, "flush_ticks": f, "max_slots": 0}
+                                          for g in (False, True) for f in (16, 64)])
+        res, st = annie_amd.batch_run(batch, engine="gpu", time_limit=args.class_seconds, **cfg)
         busy = st["seconds"] - st["setup_seconds"]
         rate_b = st["squarings"] / busy if busy > 0 else float("nan")
         bad = [r for r in res if r["status"] not in ("success", "aborted", "factor")]
@@ -232,7 +238,8 @@ def bench_proth_batch(args):
         done = sum(r["status"] == "success" for r in res)
         rows.append({"K": key[0], "pck": key[1], "LN": key[2], "lbase": key[3], "tests": len(ct),
                      "square_muls": work, "batched_sq_per_s": round(rate_b, 1), "plain_sq_per_s": round(rate_p, 1),
-                     "speedup": round(rate_b / rate_p, 2), "slots": st["slots"], "finished_in_window": done})
+                     "speedup": round(rate_b / rate_p, 2), "slots": st["slots"], "finished_in_window": done,
+                     "schedule": cfg})
         progress(f"class K={key[0]} LN={key[2]} lbase={key[3]}: {len(ct)} tests, batched {rate_b:.0f} sq/s "
                  f"({st['slots']} slots, {done} finished), plain {rate_p:.0f} sq/s")
     tph_b = len(tests) / hours_b
@@ -243,7 +250,7 @@ def bench_proth_batch(args):
         "plain_tests_per_hour": round(tph_p, 1), "speedup_vs_plain": round(tph_b / tph_p, 2),
         "dtype": "u32", "data": "synthetic",
         "config": {"workload": "4,096 Proth k*2^n+1, k odd in [3, 2^20), n log-uniform [2e4, 4e5], seed 4, Euler test",
-                   "class_seconds": args.class_seconds, "max_slots": args.max_slots},
+                   "class_seconds": args.class_seconds, "max_slots": args.max_slots, "autotune": args.autotune},
         "classes": rows,
     }
 
@@ -330,6 +337,9 @@ def main():
     ap.add_argument("--class-seconds", type=float, default=3.0)
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--max-slots", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true", help="proth-batch: per-class schedule from the autotuner")
+    ap.add_argument("--tune-seconds", type=float, default=0.25)
+    ap.add_argument("--tune-cache", default=os.path.join(ROOT, "gpurun_out", "batch_tune.json"))
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--calib-seconds", type=float, default=5.0)
     args = ap.parse_args()

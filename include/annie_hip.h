@@ -143,6 +143,10 @@ int annie_hip_batch_mul(annie_hip_t* b, size_t dst, size_t src, const uint32_t* 
 int annie_hip_batch_get(annie_hip_t* b, size_t reg, const uint32_t* slots, size_t count, uint64_t* limbs, size_t stride,
                         size_t* nout);
 int annie_hip_batch_sync(annie_hip_t* b);
+/* on != 0: batched squarings replay captured HIP graphs (one per slot-list
+ * length, captured on first use) instead of issuing their launches one by
+ * one; results are identical either way. */
+int annie_hip_batch_set_graphs(annie_hip_t* b, int on);
 
 /* Releases the context (the reference deletes the transform, src/annie.h:406). */
 void annie_hip_destroy(annie_hip_t* ctx);

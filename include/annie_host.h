@@ -73,6 +73,8 @@ typedef struct {
   double time_limit;          /* > 0: stop after that many seconds, not counting batch set-up
                                  (unfinished tests: status 2) */
   int verbose;                /* print each result line on stderr as it is decided */
+  int graphs;                 /* 1: batched squarings replay captured HIP graphs */
+  int flush_ticks;            /* deferred Gerbicz-Li products taken every that many ticks, 0 = 16 */
 } annie_batch_opts;
 
 typedef struct {

@@ -80,3 +80,16 @@ def test_time_limit_aborts_unfinished():
     res, st = annie_amd.batch_run(tests, engine="gmp", time_limit=0.2)
     assert res[0]["status"] == "aborted"
     assert 0 < res[0]["iterations"] < 20000
+
+
+def test_autotuner_picks_a_measured_config(tmp_path):
+    from annie_amd import autotune
+    tests = _mix(13, 4)
+    cfgs = [{"graphs": False, "flush_ticks": f, "max_slots": 0} for f in (4, 16)]
+    best, scores = autotune.tune_class(tests, seconds=0.05, configs=cfgs, engine="gmp")
+    assert best in cfgs and len(scores) == 2
+    path = str(tmp_path / "tune.json")
+    b1 = autotune.tuned((4, 2, 10, 28), tests, path=path, seconds=0.05, configs=cfgs, engine="gmp")
+    assert autotune.load_cache(path)[autotune.key_of((4, 2, 10, 28))]["best"] == b1
+    # cached: no re-measurement needed
+    assert autotune.tuned((4, 2, 10, 28), tests, path=path, configs=[{"bogus": 1}], engine="gmp") == b1

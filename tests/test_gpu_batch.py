@@ -52,6 +52,14 @@ def test_batch_fermat_mix_matches_powm(gpu):
         assert r["res64"] == oracle.powm(3, t["N"] - 1, t["N"]) & (2 ** 64 - 1)
 
 
+def test_graph_replays_match_launches(gpu):
+    tests = _proth_mix(45, 10, 1500, 5000)
+    a, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3)
+    b, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3, graphs=True)
+    for x, y in zip(a, b):
+        assert (x["status"], x["res64"], x["is_prp"], x["gl_ok"]) == (y["status"], y["res64"], y["is_prp"], y["gl_ok"])
+
+
 def test_same_result_alone_and_in_the_mix(gpu):
     mix = _proth_mix(43, 12, 2000, 6000)
     alone = [annie_amd.batch_run([t], engine="gpu")[0][0] for t in mix[:3]]
@@ -85,9 +93,13 @@ def _batch_for(Ns, a):
     return lib, h
 
 
-def test_batch_abi_per_slot_multipliers(gpu):
+@pytest.mark.parametrize("graphs", [False, True])
+def test_batch_abi_per_slot_multipliers(gpu, graphs):
     """r0 = a_s * r0^2 per slot, with a different multiplier pattern in each
-    slot: after t steps slot s holds 3^(its bit string) mod N_s."""
+    slot: after t steps slot s holds 3^(its bit string) mod N_s.  With graphs
+    the squarings are replays of one captured graph; the values are read back
+    (eager work and a synchronisation) between groups of replays and checked
+    at every read."""
     rng = random.Random(44)
     Ns = []
     while len(Ns) < 6:
@@ -100,27 +112,46 @@ def test_batch_abi_per_slot_multipliers(gpu):
                 Ns.append(N)
     lib, h = _batch_for(Ns, 3)
     u32 = ctypes.c_uint32
+    stride = (30000 // 64) + 4
+    out = (ctypes.c_uint64 * (6 * stride))()
+    nout = (ctypes.c_size_t * 6)()
+
+    def values(ids, n):
+        assert lib.annie_hip_batch_get(h, 0, ids, n, out, stride, nout) == 0, annie_amd._err(lib)
+        return [sum(int(v) << (64 * i) for i, v in enumerate(out[j * stride:j * stride + nout[j]])) for j in range(n)]
+
     try:
+        if graphs:
+            assert lib.annie_hip_batch_set_graphs(h, 1) == 0, annie_amd._err(lib)
         ids = (u32 * 6)(*range(6))
         assert lib.annie_hip_batch_set1(h, ids, 6) == 0
         bits = [[rng.randrange(2) for _ in range(40)] for _ in range(6)]
         for step in range(40):
             a = (u32 * 6)(*[3 if bits[s][step] else 1 for s in range(6)])
             assert lib.annie_hip_batch_square_mul(h, ids, a, 6) == 0, annie_amd._err(lib)
-        # a sub-list: only slots 1 and 4 take one more squaring
+            if step % 10 == 9:
+                for s, v in enumerate(values(ids, 6)):
+                    e = int("".join(map(str, bits[s][:step + 1])), 2)
+                    assert v == pow(3, e, Ns[s]), f"slot {s} after {step + 1} steps"
+        # a sub-list: only slots 4 and 1 take one more squaring
         sub = (u32 * 2)(4, 1)
         assert lib.annie_hip_batch_square_mul(h, sub, (u32 * 2)(3, 3), 2) == 0
-        stride = (30000 // 64) + 4
-        out = (ctypes.c_uint64 * (6 * stride))()
-        nout = (ctypes.c_size_t * 6)()
-        assert lib.annie_hip_batch_get(h, 0, ids, 6, out, stride, nout) == 0, annie_amd._err(lib)
-        for s, N in enumerate(Ns):
+        for s, v in enumerate(values(ids, 6)):
             e = int("".join(map(str, bits[s])), 2)
             if s in (1, 4):
                 e = 2 * e + 1
-            limbs = out[s * stride:s * stride + nout[s]]
-            got = sum(int(v) << (64 * i) for i, v in enumerate(limbs))
-            assert got == pow(3, e, N), f"slot {s}"
+            assert v == pow(3, e, Ns[s]), f"slot {s}"
+        # snapshot flag: slot 2 squares with bit 31 set, then r1 = r1 * snapshot
+        r0 = values(ids, 6)
+        assert lib.annie_hip_batch_copy(h, 1, 0, ids, 6) == 0
+        flag = (u32 * 6)(*[(1 | 0x80000000) if s == 2 else 1 for s in range(6)])
+        assert lib.annie_hip_batch_square_mul(h, ids, flag, 6) == 0
+        assert lib.annie_hip_batch_square_mul(h, ids, (u32 * 6)(*[1] * 6), 6) == 0
+        one = (u32 * 1)(2)
+        assert lib.annie_hip_batch_mul(h, 1, 3, one, 1) == 0, annie_amd._err(lib)
+        assert lib.annie_hip_batch_get(h, 1, one, 1, out, stride, nout) == 0
+        got = sum(int(v) << (64 * i) for i, v in enumerate(out[:nout[0]]))
+        assert got == r0[2] * pow(r0[2], 2, Ns[2]) % Ns[2]
         # argument checks: duplicate slot, slot out of range, empty list
         assert lib.annie_hip_batch_set1(h, (u32 * 2)(1, 1), 2) != 0
         assert lib.annie_hip_batch_set1(h, (u32 * 1)(6), 1) != 0
