@@ -400,37 +400,6 @@ static int copy_reg(Ctx& c, int dst, int src) {
   return ANNIE_OK;
 }
 
-// Coefficients of x in base B = 2^cbits (bits [i*cbits, (i+1)*cbits) of x),
-// as K residue planes (src/transformGPU.h:323-360).
-static void mpz_to_residues(const Ctx& c, const mpz_t x, std::vector<uint32_t>& out) {
-  const size_t n = c.n;
-  const int cb = c.pl.cbits(), K = c.pl.K;
-  out.assign(size_t(K) * n, 0);
-  const size_t xs = mpz_size(x);
-  const mp_limb_t* xp = mpz_limbs_read(x);
-  auto word = [&](size_t pos, int nb) -> uint64_t {  // nb <= 64 bits starting at pos
-    const size_t li = pos / 64;
-    const int sh = int(pos % 64);
-    uint64_t lo = li < xs ? uint64_t(xp[li]) : 0, hi = (li + 1) < xs ? uint64_t(xp[li + 1]) : 0;
-    uint64_t r = lo >> sh;
-    if (sh) r |= hi << (64 - sh);
-    return nb == 64 ? r : (r & ((uint64_t(1) << nb) - 1));
-  };
-  for (size_t i = 0; i < n; ++i) {
-    const size_t pos = i * size_t(cb);
-    const int b0 = cb < 64 ? cb : 64;
-    const uint64_t lo = word(pos, b0);
-    const uint64_t hi = cb > 64 ? word(pos + 64, cb - 64) : 0;
-    for (int k = 0; k < K; ++k) {
-      const uint32_t p = PRIMES[k];
-      // (hi * 2^64 + lo) mod p
-      const uint64_t t64 = uint64_t(pow2mod(64, p));
-      const uint64_t r = ((hi % p) * t64 + (lo % p)) % p;
-      out[size_t(k) * n + i] = uint32_t(r);
-    }
-  }
-}
-
 // Signed digits (DIGIT_LIMBS planes, two's complement) -> integer sum d_i B^i.
 static void digits_to_mpz(const Ctx& c, const std::vector<uint32_t>& d, mpz_t z) {
   const size_t n = c.n;
@@ -472,10 +441,48 @@ static void digits_to_mpz(const Ctx& c, const std::vector<uint32_t>& d, mpz_t z)
 
 static uint32_t* slot_reg(Ctx& c, int r, size_t slot) { return c.reg(r) + slot * c.slot_stride; }
 
+// Coefficients of x in base B = 2^cbits as DIGIT_LIMBS planes of 32-bit limbs
+// (bits [i*cbits, (i+1)*cbits) of x, non-negative).
+static void mpz_to_digit_planes(const Ctx& c, const mpz_t x, std::vector<uint32_t>& out) {
+  const size_t n = c.n;
+  const int cb = c.pl.cbits();
+  out.assign(size_t(DIGIT_LIMBS) * n, 0);
+  const size_t xs = mpz_size(x);
+  const mp_limb_t* xp = mpz_limbs_read(x);
+  auto bit32 = [&](size_t pos, int nb) -> uint32_t {  // nb <= 32 bits starting at pos
+    const size_t li = pos / 64;
+    const int sh = int(pos % 64);
+    const uint64_t lo = li < xs ? uint64_t(xp[li]) : 0, hi = (li + 1) < xs ? uint64_t(xp[li + 1]) : 0;
+    uint64_t r = lo >> sh;
+    if (sh) r |= hi << (64 - sh);
+    return uint32_t(nb == 32 ? r : (r & ((uint64_t(1) << nb) - 1)));
+  };
+  for (size_t i = 0; i < n; ++i) {
+    const size_t pos = i * size_t(cb);
+    for (int l = 0; l < DIGIT_LIMBS && 32 * l < cb; ++l) {
+      const int nb = std::min(32, cb - 32 * l);
+      out[size_t(l) * n + i] = bit32(pos + 32 * size_t(l), nb);
+    }
+  }
+}
+
+// set_mpz (src/transformGPU.h:323-360): the host splits x into digits, the
+// device encodes them into K residue planes.
 static int upload_mpz(Ctx& c, int r, const mpz_t x, size_t slot = 0) {
   std::vector<uint32_t> h;
-  mpz_to_residues(c, x, h);
-  HIPCHK(hipMemcpyAsync(slot_reg(c, r, slot), h.data(), h.size() * 4, hipMemcpyHostToDevice, c.stream));
+  mpz_to_digit_planes(c, x, h);
+  uint32_t* digits = c.rawtmp() + slot * c.slot_stride;
+  HIPCHK(hipMemcpyAsync(digits, h.data(), h.size() * 4, hipMemcpyHostToDevice, c.stream));
+  CarryArgs A = c.cbase;
+  const dim3 grid(unsigned((c.n / CARRY_L + 255) / 256)), block(256);
+  uint32_t* dst = slot_reg(c, r, slot);
+  switch (c.pl.K) {
+    case 2: hipLaunchKernelGGL(k_digits_to_res<2>, grid, block, 0, c.stream, digits, dst, A); break;
+    case 3: hipLaunchKernelGGL(k_digits_to_res<3>, grid, block, 0, c.stream, digits, dst, A); break;
+    case 4: hipLaunchKernelGGL(k_digits_to_res<4>, grid, block, 0, c.stream, digits, dst, A); break;
+    default: hipLaunchKernelGGL(k_digits_to_res<5>, grid, block, 0, c.stream, digits, dst, A); break;
+  }
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c.stream));
   return ANNIE_OK;
 }

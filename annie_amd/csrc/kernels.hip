@@ -731,7 +731,9 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   }
 }
 
-// Raw signed digits -> K residue planes (CARRY_L digits per thread).
+// Digits (DIGIT_LIMBS planes, two's complement, |d| < 2^95) -> K residue
+// planes, CARRY_L digits per thread: the RNS encode of set_mpz
+// (src/transformGPU.h:323-360) on the device.
 template <int K>
 __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const CarryArgs A) {
   const size_t n = size_t(1) << A.LN;
@@ -747,7 +749,15 @@ __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const Car
     dig[2][l] = v.z;
     dig[3][l] = v.w;
   }
-  store_residues<K>(dig, A, dst, n, i0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint4 v;
+    v.x = digit_residue_wide(dig[0], k, A);
+    v.y = digit_residue_wide(dig[1], k, A);
+    v.z = digit_residue_wide(dig[2], k, A);
+    v.w = digit_residue_wide(dig[3], k, A);
+    *reinterpret_cast<uint4*>(dst + size_t(k) * n + i0) = v;
+  }
 }
 
 // Register copy for each slot of a list: words [0, len) at dst_off <- src_off
