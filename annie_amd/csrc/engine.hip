@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <algorithm>
 #include <memory>
@@ -90,6 +91,12 @@ struct Ctx {
   int m1 = 0, m2 = 0;
   size_t n = 0;
   hipStream_t stream = nullptr;
+  // Stream of the next kernel launch: `stream`, or `aux` while the
+  // negacyclic half of a squaring is issued (single two-pass contexts).
+  hipStream_t cur = nullptr;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool dual_stream = true;
   // One allocation of nslots_alloc slots; a slot is NREG registers of K * n
   // words followed by 2 * DIGIT_LIMBS * n words of raw digits (T_Q, get()).
   uint32_t* mem = nullptr;
@@ -175,10 +182,10 @@ struct ProfScope {
       c.ev.push_back(e);
     }
     c.rec_kind.push_back(kind);
-    (void)hipEventRecord(c.ev[2 * rec], c.stream);
+    (void)hipEventRecord(c.ev[2 * rec], c.cur);
   }
   ~ProfScope() {
-    if (rec != size_t(-1)) (void)hipEventRecord(c.ev[2 * rec + 1], c.stream);
+    if (rec != size_t(-1)) (void)hipEventRecord(c.ev[2 * rec + 1], c.cur);
   }
 };
 
@@ -219,15 +226,15 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   const size_t shm = size_t(lines) * (S + (S >> 5)) * sizeof(uint32_t);
   ProfScope ps(c, "k_row<" + std::to_string(c.m2) + "," + row_mode_name(mode) + ">x" + std::to_string(nj));
   switch (c.m2) {
-    case 5: launch_row_m<5>(mode, grid, block, shm, c.stream, A); break;
-    case 6: launch_row_m<6>(mode, grid, block, shm, c.stream, A); break;
-    case 7: launch_row_m<7>(mode, grid, block, shm, c.stream, A); break;
-    case 8: launch_row_m<8>(mode, grid, block, shm, c.stream, A); break;
-    case 9: launch_row_m<9>(mode, grid, block, shm, c.stream, A); break;
-    case 10: launch_row_m<10>(mode, grid, block, shm, c.stream, A); break;
-    case 11: launch_row_m<11>(mode, grid, block, shm, c.stream, A); break;
-    case 12: launch_row_m<12>(mode, grid, block, shm, c.stream, A); break;
-    case 13: launch_row_m<13>(mode, grid, block, shm, c.stream, A); break;
+    case 5: launch_row_m<5>(mode, grid, block, shm, c.cur, A); break;
+    case 6: launch_row_m<6>(mode, grid, block, shm, c.cur, A); break;
+    case 7: launch_row_m<7>(mode, grid, block, shm, c.cur, A); break;
+    case 8: launch_row_m<8>(mode, grid, block, shm, c.cur, A); break;
+    case 9: launch_row_m<9>(mode, grid, block, shm, c.cur, A); break;
+    case 10: launch_row_m<10>(mode, grid, block, shm, c.cur, A); break;
+    case 11: launch_row_m<11>(mode, grid, block, shm, c.cur, A); break;
+    case 12: launch_row_m<12>(mode, grid, block, shm, c.cur, A); break;
+    case 13: launch_row_m<13>(mode, grid, block, shm, c.cur, A); break;
     default: return fail(ANNIE_ERR_SIZE, "no row kernel for this length");
   }
   HIPCHK(hipGetLastError());
@@ -254,11 +261,11 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   const size_t shm = size_t(C) * S * sizeof(uint32_t);
   ProfScope ps(c, std::string(inverse ? "k_col_inv<" : "k_col_fwd<") + std::to_string(c.m1) + ">x" + std::to_string(nj));
   switch (c.m1) {
-    case 7: launch_col_m<7>(inverse, grid, block, shm, c.stream, A); break;
-    case 8: launch_col_m<8>(inverse, grid, block, shm, c.stream, A); break;
-    case 9: launch_col_m<9>(inverse, grid, block, shm, c.stream, A); break;
-    case 10: launch_col_m<10>(inverse, grid, block, shm, c.stream, A); break;
-    case 11: launch_col_m<11>(inverse, grid, block, shm, c.stream, A); break;
+    case 7: launch_col_m<7>(inverse, grid, block, shm, c.cur, A); break;
+    case 8: launch_col_m<8>(inverse, grid, block, shm, c.cur, A); break;
+    case 9: launch_col_m<9>(inverse, grid, block, shm, c.cur, A); break;
+    case 10: launch_col_m<10>(inverse, grid, block, shm, c.cur, A); break;
+    case 11: launch_col_m<11>(inverse, grid, block, shm, c.cur, A); break;
     default: return fail(ANNIE_ERR_SIZE, "no column kernel for this length");
   }
   HIPCHK(hipGetLastError());
@@ -313,7 +320,7 @@ static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_
   ProfScope ps(c, "k_carry<" + std::to_string(c.pl.K) + ">x" + std::to_string(nj));
   switch (c.pl.K * 4 + qs) {
 #define ANNIE_CARRY_CASE(K, QS) \
-  case K * 4 + QS: hipLaunchKernelGGL((k_carry<K, QS>), grid, block, 0, c.stream, A); break;
+  case K * 4 + QS: hipLaunchKernelGGL((k_carry<K, QS>), grid, block, 0, c.cur, A); break;
     ANNIE_CARRY_CASE(2, 0)
     ANNIE_CARRY_CASE(2, 1)
     ANNIE_CARRY_CASE(2, 2)
@@ -349,15 +356,18 @@ static CarryJob cjob(const uint32_t* src, uint32_t* dst, uint32_t a, int nega, i
 // reduceMMGW (src/transformGPU.h:523-538): with T_R in RTR and T_Q (raw digits)
 // in tq(): m' = T_R N^-1 mod R, canonical, + 2; dst = (m' + 2) N - T_Q mod Q.
 // snap (batches): per-slot flags; a flagged slot also gets the result in RSNAP.
-static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr) {
+// join: T_Q is being produced on the aux stream; wait for it before the last
+// carry.  The (m' + 2) N product works in RSQ, so RWQ may be in use on aux.
+static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr, bool join = false) {
   int rc;
   NttJob j = job(c.reg(RTR), c.reg(RWR), c.fwdR(), c.invR(), c.reg(RNIR));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
   CarryJob cj = cjob(c.reg(RWR), c.reg(RTR), 1u, 0, 1, 0);
   if ((rc = launch_carry(c, &cj, 1))) return rc;
-  j = job(c.reg(RTR), c.reg(RWQ), c.fwdQ(), c.invQ(), c.reg(RNQ));
+  j = job(c.reg(RTR), c.reg(RSQ), c.fwdQ(), c.invQ(), c.reg(RNQ));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
-  cj = cjob(c.reg(RWQ), dst, 1u, 1, 0, raw_out ? 1 : 0, c.tq());
+  if (join) HIPCHK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+  cj = cjob(c.reg(RSQ), dst, 1u, 1, 0, raw_out ? 1 : 0, c.tq());
   if (snap != nullptr) cj.dst2 = c.reg(RSNAP);
   return launch_carry(c, &cj, 1, nullptr, snap);
 }
@@ -366,6 +376,29 @@ static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nu
 // batch operation (device array in slot-list order), replacing a.
 static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
   int rc;
+  if (c.aux != nullptr && a_slot == nullptr && !c.profiling) {
+    // Two streams (not while per-kernel timing is on, so that each kernel is
+    // timed alone): the negacyclic half (x^2 mod Q -> T_Q) on aux runs beside
+    // the cyclic half and the m' chain on the main stream, whose one-job
+    // passes leave much of the GPU idle; the last carry joins them.
+    HIPCHK(hipEventRecord(c.ev_fork, c.stream));
+    HIPCHK(hipStreamWaitEvent(c.aux, c.ev_fork, 0));
+    c.cur = c.aux;
+    const NttJob jq = job(c.reg(R0), c.reg(RWQ), c.fwdQ(), c.invQ());
+    rc = convolve(c, ROW_SQR, &jq, 1);
+    if (rc == ANNIE_OK) {
+      const CarryJob cq = cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1);
+      rc = launch_carry(c, &cq, 1);
+    }
+    c.cur = c.stream;
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c.ev_join, c.aux));
+    const NttJob jr = job(c.reg(R0), c.reg(RWR), c.fwdR(), c.invR());
+    if ((rc = convolve(c, ROW_SQR, &jr, 1))) return rc;
+    const CarryJob cr = cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0);
+    if ((rc = launch_carry(c, &cr, 1))) return rc;
+    return reduce(c, c.reg(R0), false, nullptr, true);
+  }
   const NttJob j[2] = {job(c.reg(R0), c.reg(RWR), c.fwdR(), c.invR()), job(c.reg(R0), c.reg(RWQ), c.fwdQ(), c.invQ())};
   if ((rc = convolve(c, ROW_SQR, j, 2))) return rc;
   const CarryJob cj[2] = {cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0), cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1)};
@@ -557,6 +590,12 @@ static void destroy(annie_hip_t* h) {
     (void)hipSetDevice(c.device);
     (void)hipStreamSynchronize(c.stream);
   }
+  if (c.aux) {
+    (void)hipStreamSynchronize(c.aux);
+    (void)hipStreamDestroy(c.aux);
+  }
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
   if (c.mem) (void)hipFree(c.mem);
   if (c.tw) (void)hipFree(c.tw);
   for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
@@ -587,6 +626,15 @@ static int setup_device(Ctx& c, int device, size_t slots) {
   c.device = device;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  c.cur = c.stream;
+  // ANNIE_SINGLE_STREAM=1 keeps both halves of a squaring on one stream.
+  const char* single = std::getenv("ANNIE_SINGLE_STREAM");
+  if (single && single[0] == '1') c.dual_stream = false;
+  if (slots == 1 && c.m1 > 0 && c.dual_stream) {
+    HIPCHK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+  }
   const size_t K = size_t(c.pl.K), n = c.n;
   c.nslots_alloc = slots;
   c.slot_stride = size_t(NREG) * K * n + 2 * size_t(DIGIT_LIMBS) * n;

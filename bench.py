@@ -10,9 +10,11 @@ them.  Registers, twiddles and constants are resident in HBM before timing.
 
 Prints ONE JSON line.  `value` is ms per squaring (lower is better), timed
 with HIP events on the engine's stream over `--steps` squarings after
-`--warmup` untimed ones.  `roofline` is computed live for the kernel that
-takes the most time per squaring (HIP event pairs around its launches over a
-profile window of the same workload); `cpu_baseline` is the reference's own
+`--warmup` untimed ones (the negacyclic half of each squaring runs on a
+second stream).  `roofline` is computed live for the kernel that takes the
+most time per squaring (HIP event pairs around its launches over a profile
+window of the same workload, issued on one stream so each kernel is timed
+alone); `cpu_baseline` is the reference's own
 CPU transform (src/transformCPU.h, built from the reference sources into
 oracle/_ref) timed on a bounded sample on one host core, run concurrently
 while the GPU context initialises.
