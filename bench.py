@@ -323,6 +323,54 @@ def bench_serve(args):
     }
 
 
+def bench_proth3m(args):
+    """ms/squaring of the Euler (Proth) test of k*2^3,000,000+1 (seeded 32-bit
+    odd k), the reference's plan K = 4, LN = 16; the reference CPU transform
+    timed on a bounded sample of the same N on one core."""
+    import annie_amd
+    import oracle  # the CPU baseline leg only
+    rng = random.Random(2)
+    while True:
+        k = rng.getrandbits(32) | 1 | (1 << 31)
+        N = k * (1 << 3_000_000) + 1
+        a = euler_base(N)
+        T = annie_amd.Transform(N, a)
+        if not T.init_failed():
+            break
+        T.close()
+    e = (N - 1) >> 1
+    total = args.warmup + args.steps
+    bits = [int(c) for c in format(e >> (e.bit_length() - total), "b")]
+    T.set1()
+    T.square_mul_bits(bits[:args.warmup], a)
+    T.sync()
+    ms = T.time_square_mul_bits(bits[args.warmup:], a) / args.steps
+    plan = (T.K, T.pck, T.LN, T.lbase)
+    T.close()
+    progress(f"GPU {ms:.4f} ms/squaring; CPU baseline (reference transformCPU, one core)")
+    cpu = None
+    if not args.no_cpu_baseline:
+        kind = "reference" if oracle.ref_available() else "port"
+        C = oracle.RefTransform(N, a, 1) if kind == "reference" else oracle.NttTransform(N, a)
+        C.set1()
+        done, t0 = 0, time.time()
+        while done < len(bits) and (done < 2 or time.time() - t0 < args.cpu_seconds / 2):
+            C.square_mul(a if bits[done] else 1)
+            done += 1
+        dt = time.time() - t0
+        cpu = {"value": round(dt * 1e3 / done, 3), "unit": "ms/squaring", "cores": 1, "kind": kind,
+               "sample": f"{done} squarings of the same N ({dt:.1f} s), one core"}
+        C.close()
+    return {
+        "metric": "ms/squaring, Proth k*2^3,000,000+1 (Euler test)", "value": round(ms, 5),
+        "unit": "ms/squaring", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": False, "dtype": "u32", "data": "synthetic",
+        "config": {"workload": f"Euler test of k*2^3000000+1, k = {k} (seed 2), base {a}",
+                   "K": plan[0], "pck": plan[1], "LN": plan[2], "lbase": plan[3]},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,9 +380,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", type=int, default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=["squaring", "proth-batch", "serve"], default="squaring",
-                    help="squaring: the ~40M-digit square_mul loop (default); proth-batch: 4,096 Proth tests, "
-                         "batched vs plain; serve: Poisson arrivals with continuous batching")
+    ap.add_argument("--workload", choices=["squaring", "proth3m", "proth-batch", "serve"], default="squaring",
+                    help="squaring: the ~40M-digit square_mul loop (default); proth3m: k*2^3,000,000+1; "
+                         "proth-batch: 4,096 Proth tests, batched vs plain; serve: Poisson arrivals with "
+                         "continuous batching")
     ap.add_argument("--tests", type=int, default=4096)
     ap.add_argument("--class-seconds", type=float, default=3.0)
     ap.add_argument("--plain-steps", type=int, default=400)
@@ -356,7 +405,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     if args.workload != "squaring":
-        line = bench_proth_batch(args) if args.workload == "proth-batch" else bench_serve(args)
+        line = {"proth3m": bench_proth3m, "proth-batch": bench_proth_batch, "serve": bench_serve}[args.workload](args)
         if rank == 0:
             print(json.dumps(line), flush=True)
         return
