@@ -226,6 +226,7 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   const size_t shm = size_t(lines) * (S + (S >> 5)) * sizeof(uint32_t);
   ProfScope ps(c, "k_row<" + std::to_string(c.m2) + "," + row_mode_name(mode) + ">x" + std::to_string(nj));
   switch (c.m2) {
+    case 4: launch_row_m<4>(mode, grid, block, shm, c.cur, A); break;
     case 5: launch_row_m<5>(mode, grid, block, shm, c.cur, A); break;
     case 6: launch_row_m<6>(mode, grid, block, shm, c.cur, A); break;
     case 7: launch_row_m<7>(mode, grid, block, shm, c.cur, A); break;
@@ -733,7 +734,7 @@ static int create(const uint64_t* Nl, size_t nl, int32_t a, int device, int min_
   c.pl = plan_for(c.N, a);
   if (c.pl.LN < min_ln) return fail(ANNIE_ERR_SIZE, "NTT size < 2^" + std::to_string(min_ln) + " is not supported");
   if (c.pl.LN > 23) return fail(ANNIE_ERR_SIZE, "NTT size > 2^23 is not supported");
-  if (c.pl.LN < 5) return fail(ANNIE_ERR_SIZE, "NTT size < 2^5 is not supported by this engine");
+  if (c.pl.LN < 4) return fail(ANNIE_ERR_SIZE, "NTT size < 2^4 is not supported by this engine");
 
   // R = B^n - 1; N must be invertible mod R (src/transformGPU.h:652-661).
   mpz_t R, t;
@@ -1103,7 +1104,7 @@ int annie_hip_read_tq(annie_hip_t* h, uint32_t* out, size_t cap_words) {
 int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int device, annie_hip_t** out) {
   if (!out) return fail(ANNIE_ERR_ARG, "null argument");
   *out = nullptr;
-  if (K < 2 || K > KMAX || (pck != 1 && pck != 2 && pck != 4) || LN < 5 || LN > 23 || lbase < 2 || lbase > 31 ||
+  if (K < 2 || K > KMAX || (pck != 1 && pck != 2 && pck != 4) || LN < 4 || LN > 23 || lbase < 2 || lbase > 31 ||
       pck * lbase > 32 * DIGIT_LIMBS - 1)
     return fail(ANNIE_ERR_ARG, "bad plan");
   if (slots == 0 || slots > 32767) return fail(ANNIE_ERR_ARG, "slot count must be in [1, 32767]");

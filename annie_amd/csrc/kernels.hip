@@ -771,14 +771,15 @@ __global__ void k_copy_slots(uint32_t* base, size_t dst_off, size_t src_off, siz
     dst[i] = src[i];
 }
 
-// Explicit instantiations, looked up by the host engine: row lines of 2^5 ..
-// 2^13 elements (single-pass lengths 2^5..2^13, second pass of 2^14..2^23) and
+// Explicit instantiations, looked up by the host engine: row lines of 2^4 ..
+// 2^13 elements (single-pass lengths 2^4..2^13, second pass of 2^14..2^23) and
 // column lines of 2^7 .. 2^11 elements (first pass of 2^14..2^23).
 #define ANNIE_ROW(M)                                          \
   template __global__ void k_row<M, ROW_FWD>(const NttArgs);  \
   template __global__ void k_row<M, ROW_SQR>(const NttArgs);  \
   template __global__ void k_row<M, ROW_MULC>(const NttArgs); \
   template __global__ void k_row<M, ROW_MULV>(const NttArgs);
+ANNIE_ROW(4)
 ANNIE_ROW(5)
 ANNIE_ROW(6)
 ANNIE_ROW(7)

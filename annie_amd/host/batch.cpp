@@ -393,7 +393,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       std::snprintf(r.message, sizeof(r.message), "Error: %s.", annie_hip_last_error());
       continue;
     }
-    if (LN < 5 || LN > 23) {
+    if (LN < 4 || LN > 23) {
       r.status = -1;
       std::snprintf(r.message, sizeof(r.message), "Error: no transform of this size.");
       continue;

@@ -7,7 +7,7 @@ import numpy as np
 import oracle
 
 
-def usable_N(bits, seed, a=3, min_ln=5):
+def usable_N(bits, seed, a=3, min_ln=4):
     """Seeded random odd `bits`-bit N (top bit set) whose transform initialises.
 
     gcd(N, R) > 1 makes the reference report a factor instead of testing
