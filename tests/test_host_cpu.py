@@ -104,14 +104,15 @@ def test_checkpoint_name_closed_form():
     assert buf.value.decode() == f"a_{h:016X}.chk"
 
 
-def test_cli_expression_front_end():
+def test_cli_expression_front_end(tmp_path):
     exe = os.path.join(annie_amd.LIBDIR, "annie")
-    out = subprocess.run([exe, "-q", "fibonacci(359)", "-w", "/tmp"], capture_output=True, text=True, timeout=120)
+    w = str(tmp_path)
+    out = subprocess.run([exe, "-q", "fibonacci(359)", "-w", w], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "is a Fermat probable prime" in out.stdout
-    out = subprocess.run([exe, "-q", "(2^127-1)*3+2*0", "-w", "/tmp"], capture_output=True, text=True, timeout=120)
+    out = subprocess.run([exe, "-q", "(2^127-1)*3+2*0", "-w", w], capture_output=True, text=True, timeout=120)
     assert "composite" in out.stdout
-    out = subprocess.run([exe, "-q", "10!+1"], capture_output=True, text=True, timeout=60)
+    out = subprocess.run([exe, "-q", "10!+1"], capture_output=True, text=True, timeout=60, cwd=w)
     assert "3628801 is" in out.stdout
-    out = subprocess.run([exe, "-q", "unknown(3)"], capture_output=True, text=True, timeout=60)
+    out = subprocess.run([exe, "-q", "unknown(3)"], capture_output=True, text=True, timeout=60, cwd=w)
     assert out.returncode == 1 and "unknown function" in out.stderr
