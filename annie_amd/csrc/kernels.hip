@@ -48,44 +48,52 @@ __device__ __forceinline__ void inv_bfly(uint32_t& a, uint32_t& b, const Tw2 w, 
   b = mont_c(d, w.w, w.wq, p);
 }
 
-__device__ __forceinline__ Tw2 ldtw(const TwDev* t, uint32_t i, uint32_t q) {
-  const uint32_t w = t[i];
-  return Tw2{w, w * q};
-}
-
-// Levels [d0, d0 + RR) on one group of 2^RR register elements; H0 is the heap
-// index of the group's node at level d0 (children of heap node h are 2h, 2h+1).
+// Roots of one register round (levels [d0, d0 + RR)) for each of the
+// thread's NTT_R >> RR groups: the 2^RR - 1 roots of a group in heap order,
+// level e of the round at entries 2^e - 1 .. 2^{e+1} - 2.  A round's roots are
+// loaded one round ahead of their use, so their latency overlaps the previous
+// round's butterflies and LDS exchange.
 template <int RR>
-__device__ __forceinline__ void group_fwd(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p, uint32_t q) {
+struct RoundTw {
+  static constexpr int G = NTT_R >> RR, W = (1 << RR) - 1;
+  uint32_t w[G * W];
+};
+
+// Levels [0, RR) of one group of 2^RR register elements with the group's
+// roots w (heap order, as in RoundTw).
+template <int RR>
+__device__ __forceinline__ void group_fwd(uint32_t* x, const uint32_t* w, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int e = 0; e < RR; ++e) {
     constexpr int N = 1 << RR;
     const int half = N >> (e + 1);
 #pragma unroll
     for (int blk = 0; blk < (1 << e); ++blk) {
-      const Tw2 w = ldtw(tw, (H0 << e) + blk, q);
+      const uint32_t r = w[(1 << e) - 1 + blk];
+      const Tw2 t{r, r * q};
 #pragma unroll
       for (int jj = 0; jj < half; ++jj) {
         const int j = blk * 2 * half + jj;
-        fwd_bfly(x[j], x[j + half], w, p);
+        fwd_bfly(x[j], x[j + half], t, p);
       }
     }
   }
 }
 
 template <int RR>
-__device__ __forceinline__ void group_inv(uint32_t* x, uint32_t H0, const TwDev* tw, uint32_t p, uint32_t q) {
+__device__ __forceinline__ void group_inv(uint32_t* x, const uint32_t* w, uint32_t p, uint32_t q) {
 #pragma unroll
   for (int e = RR - 1; e >= 0; --e) {
     constexpr int N = 1 << RR;
     const int half = N >> (e + 1);
 #pragma unroll
     for (int blk = 0; blk < (1 << e); ++blk) {
-      const Tw2 w = ldtw(tw, (H0 << e) + blk, q);
+      const uint32_t r = w[(1 << e) - 1 + blk];
+      const Tw2 t{r, r * q};
 #pragma unroll
       for (int jj = 0; jj < half; ++jj) {
         const int j = blk * 2 * half + jj;
-        inv_bfly(x[j], x[j + half], w, p);
+        inv_bfly(x[j], x[j + half], t, p);
       }
     }
   }
@@ -131,26 +139,33 @@ __device__ __forceinline__ void round_store(const uint32_t (&x)[NTT_R], int m, i
   }
 }
 
+// Load the roots of round (d0, RR) of the line whose root node is lineRoot
+// (heap index at level 0 of this pass).
 template <int RR>
-__device__ __forceinline__ void round_fwd(uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, uint32_t lineRoot,
-                                          const TwDev* tw, uint32_t p, uint32_t q) {
+__device__ __forceinline__ void round_tw(RoundTw<RR>& T, int m, int d0, uint32_t tg, uint32_t lineRoot,
+                                         const TwDev* tw) {
 #pragma unroll
-  for (int s = 0; s < (NTT_R >> RR); ++s) {
+  for (int s = 0; s < RoundTw<RR>::G; ++s) {
     uint32_t node;
     group_base<RR>(m, d0, tg, s, node);
-    group_fwd<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p, q);
+    const uint32_t H0 = (lineRoot << d0) + node;
+#pragma unroll
+    for (int e = 0; e < RR; ++e)
+#pragma unroll
+      for (int blk = 0; blk < (1 << e); ++blk) T.w[s * RoundTw<RR>::W + (1 << e) - 1 + blk] = tw[(H0 << e) + blk];
   }
 }
 
 template <int RR>
-__device__ __forceinline__ void round_inv(uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, uint32_t lineRoot,
-                                          const TwDev* tw, uint32_t p, uint32_t q) {
+__device__ __forceinline__ void round_fwd(uint32_t (&x)[NTT_R], const RoundTw<RR>& T, uint32_t p, uint32_t q) {
 #pragma unroll
-  for (int s = 0; s < (NTT_R >> RR); ++s) {
-    uint32_t node;
-    group_base<RR>(m, d0, tg, s, node);
-    group_inv<RR>(&x[s * (1 << RR)], (lineRoot << d0) + node, tw, p, q);
-  }
+  for (int s = 0; s < RoundTw<RR>::G; ++s) group_fwd<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<RR>::W], p, q);
+}
+
+template <int RR>
+__device__ __forceinline__ void round_inv(uint32_t (&x)[NTT_R], const RoundTw<RR>& T, uint32_t p, uint32_t q) {
+#pragma unroll
+  for (int s = 0; s < RoundTw<RR>::G; ++s) group_inv<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<RR>::W], p, q);
 }
 
 // Round schedule of a line of 2^M elements: rounds of four levels, the last
@@ -163,36 +178,70 @@ struct Sched {
   static constexpr int rr() { return (M - 4 * I) < 4 ? (M - 4 * I) : 4; }
 };
 
+template <int M, int I>
+using SchedTw = RoundTw<Sched<M>::template rr<I>()>;
+
 // Forward rounds I.. of a line: round 0 loads through gld, later rounds
 // through LDS; every round but the last stores to LDS, and the last leaves its
-// elements in registers.
-template <int M, int I, class GLd, class LLd, class LSt>
+// elements in registers.  With AHEAD, round I's roots arrive in `pre` and each
+// round first issues the loads of the next round's roots (so their latency
+// overlaps a round of butterflies and the LDS exchange); otherwise each round
+// loads its own roots (from a table in LDS) and `pre` is unused.  `at_last`
+// runs at the start of the last round (the caller's prefetches for what
+// follows).  With PRE, round 0's elements are already in x.
+template <int M, int I, bool AHEAD, bool PRE, class GLd, class LLd, class LSt, class Hook>
 __device__ __forceinline__ void fwd_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
-                                           uint32_t p, uint32_t q, GLd& gld, LLd& lld, LSt& lst) {
+                                           uint32_t p, uint32_t q, GLd& gld, LLd& lld, LSt& lst,
+                                           const SchedTw<M, I>& pre, Hook&& at_last) {
   constexpr int RR = Sched<M>::template rr<I>();
-  if constexpr (I == 0) round_load<RR>(x, M, 4 * I, tg, gld);
-  else round_load<RR>(x, M, 4 * I, tg, lld);
-  round_fwd<RR>(x, M, 4 * I, tg, lineRoot, tw, p, q);
+  SchedTw<M, I> own;
+  if constexpr (!AHEAD) round_tw(own, M, 4 * I, tg, lineRoot, tw);
+  const SchedTw<M, I>& cur = AHEAD ? pre : own;
   if constexpr (I + 1 < Sched<M>::NR) {
+    SchedTw<M, I + 1> nxt;
+    if constexpr (AHEAD) round_tw(nxt, M, 4 * (I + 1), tg, lineRoot, tw);
+    if constexpr (I == 0) {
+      if constexpr (!PRE) round_load<RR>(x, M, 4 * I, tg, gld);
+    } else {
+      round_load<RR>(x, M, 4 * I, tg, lld);
+    }
+    round_fwd<RR>(x, cur, p, q);
     round_store<RR>(x, M, 4 * I, tg, lst);
     __syncthreads();
-    fwd_rounds<M, I + 1>(x, tg, lineRoot, tw, p, q, gld, lld, lst);
+    fwd_rounds<M, I + 1, AHEAD, PRE>(x, tg, lineRoot, tw, p, q, gld, lld, lst, nxt, at_last);
+  } else {
+    at_last();
+    if constexpr (I == 0) {
+      if constexpr (!PRE) round_load<RR>(x, M, 4 * I, tg, gld);
+    } else {
+      round_load<RR>(x, M, 4 * I, tg, lld);
+    }
+    round_fwd<RR>(x, cur, p, q);
   }
 }
 
 // Inverse rounds I down to 0, starting (I = NR - 1) from the registers of the
-// last forward round; round 0 stores through gst.
-template <int M, int I, class GSt, class LLd, class LSt>
+// last forward round; round 0 stores through gst.  Roots as in fwd_rounds:
+// with AHEAD each round issues the loads of the next (lower) round's roots.
+template <int M, int I, bool AHEAD, class GSt, class LLd, class LSt>
 __device__ __forceinline__ void inv_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
-                                           uint32_t p, uint32_t q, GSt& gst, LLd& lld, LSt& lst) {
+                                           uint32_t p, uint32_t q, GSt& gst, LLd& lld, LSt& lst,
+                                           const SchedTw<M, I>& pre) {
   constexpr int RR = Sched<M>::template rr<I>();
-  if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
-  round_inv<RR>(x, M, 4 * I, tg, lineRoot, tw, p, q);
+  SchedTw<M, I> own;
+  if constexpr (!AHEAD) round_tw(own, M, 4 * I, tg, lineRoot, tw);
+  const SchedTw<M, I>& cur = AHEAD ? pre : own;
   if constexpr (I > 0) {
+    SchedTw<M, I - 1> nxt;
+    if constexpr (AHEAD) round_tw(nxt, M, 4 * (I - 1), tg, lineRoot, tw);
+    if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
+    round_inv<RR>(x, cur, p, q);
     round_store<RR>(x, M, 4 * I, tg, lst);
     __syncthreads();
-    inv_rounds<M, I - 1>(x, tg, lineRoot, tw, p, q, gst, lld, lst);
+    inv_rounds<M, I - 1, AHEAD>(x, tg, lineRoot, tw, p, q, gst, lld, lst, nxt);
   } else {
+    if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
+    round_inv<RR>(x, cur, p, q);
     round_store<RR>(x, M, 0, tg, gst);
   }
 }
@@ -229,30 +278,43 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
 
   uint32_t x[NTT_R];
-  fwd_rounds<M, 0>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst);
+  SchedTw<M, 0> t0;
+  round_tw(t0, M, 0, tg, lineRoot, twf);
   if constexpr (MODE == ROW_FWD) {
+    fwd_rounds<M, 0, true, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
     round_store<RL>(x, M, DL, tg, gst);
   } else {
-    // Pointwise product on the leaves held in registers.
-    const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
-    const uint32_t* op = (MODE == ROW_SQR) ? nullptr : J.opnd + base;
+    // The last forward round also issues the loads of the first inverse
+    // round's roots and of the pointwise operand (MULC / MULV).
+    SchedTw<M, NR - 1> ti;
+    uint32_t opv[NTT_R];
+    fwd_rounds<M, 0, true, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
+      round_tw(ti, M, DL, tg, lineRoot, twi);
+      if constexpr (MODE != ROW_SQR) {
+        const uint32_t* op = J.opnd + base;
 #pragma unroll
-    for (int s = 0; s < (NTT_R >> RL); ++s) {
-      uint32_t node;
-      const uint32_t b = group_base<RL>(M, DL, tg, s, node);
+        for (int s = 0; s < (NTT_R >> RL); ++s) {
+          uint32_t node;
+          const uint32_t b = group_base<RL>(M, DL, tg, s, node);
 #pragma unroll
-      for (int j = 0; j < (1 << RL); ++j) {
-        uint32_t& v = x[s * (1 << RL) + j];
-        if constexpr (MODE == ROW_SQR) {
-          v = mont_c(mont(v, v, p, q), sc, scq, p);
-        } else if constexpr (MODE == ROW_MULC) {
-          v = mont(v, op[b + uint32_t(j)], p, q);
-        } else {
-          v = mont_c(mont(v, op[b + uint32_t(j)], p, q), sc, scq, p);
+          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
         }
       }
+    });
+    // Pointwise product on the leaves held in registers.
+    const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
+#pragma unroll
+    for (int i = 0; i < NTT_R; ++i) {
+      uint32_t& v = x[i];
+      if constexpr (MODE == ROW_SQR) {
+        v = mont_c(mont(v, v, p, q), sc, scq, p);
+      } else if constexpr (MODE == ROW_MULC) {
+        v = mont(v, opv[i], p, q);
+      } else {
+        v = mont_c(mont(v, opv[i], p, q), sc, scq, p);
+      }
     }
-    inv_rounds<M, NR - 1>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst);
+    inv_rounds<M, NR - 1, true>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
   }
 }
 
@@ -269,10 +331,14 @@ __device__ __forceinline__ uint32_t col_group() {
 
 // Column passes: the first M levels on columns of stride n2 = n / 2^M; `cols`
 // consecutive columns per block, interleaved in LDS so that global accesses
-// of a wavefront touch consecutive words.  Grid: (n2 / cols, K, jobs).
+// of a wavefront touch consecutive words.  All columns use the same 2^M - 1
+// roots (heap entries 1 .. 2^M - 1 of the table), which the block stages in
+// LDS after the column data while the data loads are in flight.
+// Grid: (n2 / cols, K, jobs); dynamic LDS (cols + 1) * 2^M words.
 template <int M>
-__global__ void __launch_bounds__(1024) k_col_fwd(const NttArgs A) {
+__global__ void __launch_bounds__(1024, 8) k_col_fwd(const NttArgs A) {
   extern __shared__ uint32_t lds[];
+  constexpr uint32_t S = 1u << M;
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
@@ -282,25 +348,33 @@ __global__ void __launch_bounds__(1024) k_col_fwd(const NttArgs A) {
   const size_t n = size_t(1) << A.LN;
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
-  const size_t n2 = n >> M;
-  const size_t base = so + size_t(k) * n + size_t(col_group()) * C + line;
+  // Block-uniform base pointers and 32-bit per-thread offsets (line + i n2 <
+  // 2^32), so global accesses take a scalar base and a 32-bit vector offset.
+  const uint32_t n2 = uint32_t(n >> M);
+  const size_t base = so + size_t(k) * n + size_t(col_group()) * C;
   const TwDev* twf = J.fwd + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
+  uint32_t* roots = lds + C * S;
 
-  auto gld = [&](uint32_t i) { return src[size_t(i) * n2]; };
-  auto gst = [&](uint32_t i, uint32_t v) { dst[size_t(i) * n2] = v; };
+  auto gld = [&](uint32_t i) { return src[line + i * n2]; };
+  auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
   auto lld = [&](uint32_t i) { return lds[i * C + line]; };
   auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
 
   uint32_t x[NTT_R];
-  fwd_rounds<M, 0>(x, tg, 1u, twf, p, A.q[k], gld, lld, lst);
+  round_load<Sched<M>::template rr<0>()>(x, M, 0, tg, gld);
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twf[i];
+  __syncthreads();
+  SchedTw<M, 0> unused;
+  fwd_rounds<M, 0, false, true>(x, tg, 1u, roots, p, A.q[k], gld, lld, lst, unused, [] {});
   round_store<RL>(x, M, DL, tg, gst);
 }
 
 template <int M>
-__global__ void __launch_bounds__(1024) k_col_inv(const NttArgs A) {
+__global__ void __launch_bounds__(1024, 8) k_col_inv(const NttArgs A) {
   extern __shared__ uint32_t lds[];
+  constexpr uint32_t S = 1u << M;
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
@@ -310,20 +384,26 @@ __global__ void __launch_bounds__(1024) k_col_inv(const NttArgs A) {
   const size_t n = size_t(1) << A.LN;
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
-  const size_t n2 = n >> M;
-  const size_t base = so + size_t(k) * n + size_t(col_group()) * C + line;
+  // Block-uniform base pointers and 32-bit per-thread offsets (line + i n2 <
+  // 2^32), so global accesses take a scalar base and a 32-bit vector offset.
+  const uint32_t n2 = uint32_t(n >> M);
+  const size_t base = so + size_t(k) * n + size_t(col_group()) * C;
   const TwDev* twi = J.inv + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
+  uint32_t* roots = lds + C * S;
 
-  auto gld = [&](uint32_t i) { return src[size_t(i) * n2]; };
-  auto gst = [&](uint32_t i, uint32_t v) { dst[size_t(i) * n2] = v; };
+  auto gld = [&](uint32_t i) { return src[line + i * n2]; };
+  auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
   auto lld = [&](uint32_t i) { return lds[i * C + line]; };
   auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
 
   uint32_t x[NTT_R];
   round_load<RL>(x, M, DL, tg, gld);
-  inv_rounds<M, NR - 1>(x, tg, 1u, twi, p, A.q[k], gst, lld, lst);
+  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twi[i];
+  __syncthreads();
+  SchedTw<M, NR - 1> unused;
+  inv_rounds<M, NR - 1, false>(x, tg, 1u, roots, p, A.q[k], gst, lld, lst, unused);
 }
 
 // Elementwise Montgomery scaling x -> x * s * 2^-32 of K planes (used once at
@@ -544,43 +624,53 @@ __device__ __forceinline__ void store_residues(const uint32_t (&dig)[CARRY_L][DI
   }
 }
 
-// Phase 1 for block b: digits with zero carry-in and the carry-out f.
-template <int K, int L, int QS>
-__device__ __forceinline__ void carry_block_local(const CarryJob& J, const CarryArgs& A, size_t n, size_t b, uint32_t s,
-                                                  uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], uint32_t (&f)[L]) {
-  const size_t i0 = b * CARRY_L;
+// Inputs of one block: the K residues of its CARRY_L coefficients and, for a
+// subtracting carry, the raw digits of the subtrahend.
+template <int K>
+struct CarryIn {
   uint32_t res[K][CARRY_L];
+  uint32_t sd[DIGIT_LIMBS][CARRY_L];
+};
+
+template <int K>
+__device__ __forceinline__ void carry_load(const CarryJob& J, size_t n, size_t b, CarryIn<K>& in) {
+  const size_t i0 = b * CARRY_L;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const uint4 v = *reinterpret_cast<const uint4*>(J.src + size_t(k) * n + i0);
-    res[k][0] = v.x;
-    res[k][1] = v.y;
-    res[k][2] = v.z;
-    res[k][3] = v.w;
+    in.res[k][0] = v.x;
+    in.res[k][1] = v.y;
+    in.res[k][2] = v.z;
+    in.res[k][3] = v.w;
   }
-  uint32_t sd[DIGIT_LIMBS][CARRY_L];
   if (J.sub != nullptr) {
 #pragma unroll
     for (int l = 0; l < DIGIT_LIMBS; ++l) {
       const uint4 v = *reinterpret_cast<const uint4*>(J.sub + size_t(l) * n + i0);
-      sd[l][0] = v.x;
-      sd[l][1] = v.y;
-      sd[l][2] = v.z;
-      sd[l][3] = v.w;
+      in.sd[l][0] = v.x;
+      in.sd[l][1] = v.y;
+      in.sd[l][2] = v.z;
+      in.sd[l][3] = v.w;
     }
   }
+}
+
+// Phase 1 on loaded inputs: digits with zero carry-in and the carry-out f.
+template <int K, int L, int QS>
+__device__ __forceinline__ void carry_phase1(const CarryJob& J, const CarryArgs& A, uint32_t s, const CarryIn<K>& in,
+                                             uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], uint32_t (&f)[L]) {
 #pragma unroll
   for (int i = 0; i < L; ++i) f[i] = 0u;
 #pragma unroll
   for (int j = 0; j < CARRY_L; ++j) {
     uint32_t r[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) r[k] = res[k][j];
+    for (int k = 0; k < K; ++k) r[k] = in.res[k][j];
     uint32_t v[L];
     garner<K, L>(r, A, v);
     if (J.a != 1u) big_mad<L>(v, J.a, 0u);
     if (J.sub != nullptr) {
-      const uint32_t d[DIGIT_LIMBS] = {sd[0][j], sd[1][j], sd[2][j]};
+      const uint32_t d[DIGIT_LIMBS] = {in.sd[0][j], in.sd[1][j], in.sd[2][j]};
       uint32_t e[L];
       big_from_digit<L>(d, e);
       big_sub<L>(v, e);
@@ -588,6 +678,15 @@ __device__ __forceinline__ void carry_block_local(const CarryJob& J, const Carry
     big_add<L>(f, v);
     big_split<L, QS>(f, s, dig[j]);
   }
+}
+
+// Phase 1 for block b.
+template <int K, int L, int QS>
+__device__ __forceinline__ void carry_block_local(const CarryJob& J, const CarryArgs& A, size_t n, size_t b, uint32_t s,
+                                                  uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], uint32_t (&f)[L]) {
+  CarryIn<K> in;
+  carry_load<K>(J, n, b, in);
+  carry_phase1<K, L, QS>(J, A, s, in, dig, f);
 }
 
 // Phase 2: add the carry-in f to the block's digits.
@@ -666,7 +765,33 @@ __device__ int number_sign(const CarryJob& J, const CarryArgs& A, size_t n, uint
   return 0;
 }
 
+// One finished block to the destination: raw digit planes, or K residue
+// planes (and the second destination of a flagged slot).
+template <int K>
+__device__ __forceinline__ void carry_store(const CarryJob& J, const CarryArgs& A, size_t n, size_t b,
+                                            const uint32_t (&dig)[CARRY_L][DIGIT_LIMBS]) {
+  const size_t i0 = b * CARRY_L;
+  if (J.raw_out) {
+#pragma unroll
+    for (int l = 0; l < DIGIT_LIMBS; ++l) {
+      uint4 v;
+      v.x = dig[0][l];
+      v.y = dig[1][l];
+      v.z = dig[2][l];
+      v.w = dig[3][l];
+      *reinterpret_cast<uint4*>(J.dst + size_t(l) * n + i0) = v;
+    }
+  } else {
+    store_residues<K>(dig, A, J.dst, n, i0);
+    if (J.dst2 != nullptr) store_residues<K>(dig, A, J.dst2, n, i0);
+  }
+}
+
 // Grid: (ceil(nb / (CARRY_WG - 1)), jobs), nb = n / CARRY_L blocks.
+// Workgroup w owns the CARRY_WG - 1 blocks [w (CARRY_WG - 1), (w + 1)
+// (CARRY_WG - 1)); thread t > 0 takes block w (CARRY_WG - 1) + t - 1 and
+// thread 0 the block before them (the last block for w = 0), only to hand
+// its zero-in carry to thread 1 through LDS.
 template <int K, int QS>
 __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   constexpr int L = K + 1;
@@ -714,21 +839,7 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
   }
 
-  const size_t i0 = b * CARRY_L;
-  if (J.raw_out) {
-#pragma unroll
-    for (int l = 0; l < DIGIT_LIMBS; ++l) {
-      uint4 v;
-      v.x = dig[0][l];
-      v.y = dig[1][l];
-      v.z = dig[2][l];
-      v.w = dig[3][l];
-      *reinterpret_cast<uint4*>(J.dst + size_t(l) * n + i0) = v;
-    }
-  } else {
-    store_residues<K>(dig, A, J.dst, n, i0);
-    if (J.dst2 != nullptr) store_residues<K>(dig, A, J.dst2, n, i0);
-  }
+  carry_store<K>(J, A, n, b, dig);
 }
 
 // Digits (DIGIT_LIMBS planes, two's complement, |d| < 2^95) -> K residue
