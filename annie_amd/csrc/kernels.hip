@@ -252,7 +252,7 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 // line, then (by mode) the pointwise product and the inverse row levels.
 // Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / NTT_R threads.
 template <int M, int MODE>
-__global__ void __launch_bounds__(512) k_row(const NttArgs A) {
+__global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr uint32_t S = 1u << M, Tl = S / NTT_R;
   constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
@@ -277,19 +277,23 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
   auto lld = [&](uint32_t i) { return X[lpad(i)]; };
   auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
 
+  // Roots one round ahead, except for rows of 2^12 (three full radix-16
+  // rounds), where the extra registers would leave three waves per SIMD
+  // instead of four.
+  constexpr bool AH = (M != 12);
   uint32_t x[NTT_R];
   SchedTw<M, 0> t0;
-  round_tw(t0, M, 0, tg, lineRoot, twf);
+  if constexpr (AH) round_tw(t0, M, 0, tg, lineRoot, twf);
   if constexpr (MODE == ROW_FWD) {
-    fwd_rounds<M, 0, true, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
+    fwd_rounds<M, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
     round_store<RL>(x, M, DL, tg, gst);
   } else {
-    // The last forward round also issues the loads of the first inverse
-    // round's roots and of the pointwise operand (MULC / MULV).
+    // With AH the last forward round also issues the loads of the first
+    // inverse round's roots and of the pointwise operand (MULC / MULV);
+    // otherwise the operand is loaded just before the product.
     SchedTw<M, NR - 1> ti;
     uint32_t opv[NTT_R];
-    fwd_rounds<M, 0, true, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
-      round_tw(ti, M, DL, tg, lineRoot, twi);
+    auto load_opv = [&] {
       if constexpr (MODE != ROW_SQR) {
         const uint32_t* op = J.opnd + base;
 #pragma unroll
@@ -300,7 +304,14 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
           for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
         }
       }
+    };
+    fwd_rounds<M, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
+      if constexpr (AH) {
+        round_tw(ti, M, DL, tg, lineRoot, twi);
+        load_opv();
+      }
     });
+    if constexpr (!AH) load_opv();
     // Pointwise product on the leaves held in registers.
     const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
 #pragma unroll
@@ -314,7 +325,7 @@ __global__ void __launch_bounds__(512) k_row(const NttArgs A) {
         v = mont_c(mont(v, opv[i], p, q), sc, scq, p);
       }
     }
-    inv_rounds<M, NR - 1, true>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
+    inv_rounds<M, NR - 1, AH>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
   }
 }
 
