@@ -250,16 +250,19 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   A.njobs = nj;
   A.m = c.m1;
   A.m1 = c.m1;
-  const uint32_t S = 1u << c.m1, Tl = S / NTT_R;
+  // Columns of 2^10 and 2^11: 32 elements per thread, 8 columns per block,
+  // roots from the global table.  Shorter columns: 16 elements per thread, up
+  // to 16 columns per block, roots staged in LDS after the column data.
+  const bool wide = c.m1 >= 10;
+  const uint32_t S = 1u << c.m1, Tl = S / (wide ? 32u : uint32_t(NTT_R));
   const uint32_t n2 = uint32_t(c.n >> c.m1);
-  // Up to 16 columns of Tl threads each per block ((C + 1) * 2^m1 * 4 bytes of LDS).
-  uint32_t C = 1024u / Tl;
+  uint32_t C = wide ? 8u : 1024u / Tl;
   if (C < 1) C = 1;
   if (C > 16) C = 16;
   if (C > n2) C = n2;
   A.cols = int(C);
   const dim3 grid(n2 / C, c.pl.K, nj * c.nslots), block(C * Tl);
-  const size_t shm = size_t(C + 1) * S * sizeof(uint32_t);  // column data, then the pass's roots
+  const size_t shm = size_t(wide ? C : C + 1) * S * sizeof(uint32_t);
   ProfScope ps(c, std::string(inverse ? "k_col_inv<" : "k_col_fwd<") + std::to_string(c.m1) + ">x" + std::to_string(nj));
   switch (c.m1) {
     case 7: launch_col_m<7>(inverse, grid, block, shm, c.cur, A); break;

@@ -49,13 +49,13 @@ __device__ __forceinline__ void inv_bfly(uint32_t& a, uint32_t& b, const Tw2 w, 
 }
 
 // Roots of one register round (levels [d0, d0 + RR)) for each of the
-// thread's NTT_R >> RR groups: the 2^RR - 1 roots of a group in heap order,
+// thread's 2^LR >> RR groups: the 2^RR - 1 roots of a group in heap order,
 // level e of the round at entries 2^e - 1 .. 2^{e+1} - 2.  A round's roots are
 // loaded one round ahead of their use, so their latency overlaps the previous
 // round's butterflies and LDS exchange.
-template <int RR>
+template <int LR, int RR>
 struct RoundTw {
-  static constexpr int G = NTT_R >> RR, W = (1 << RR) - 1;
+  static constexpr int G = (1 << LR) >> RR, W = (1 << RR) - 1;
   uint32_t w[G * W];
 };
 
@@ -101,38 +101,39 @@ __device__ __forceinline__ void group_inv(uint32_t* x, const uint32_t* w, uint32
 
 // Position inside the line of element jj of sub-group s held by thread slot tg
 // in a round covering levels [d0, d0 + RR) of a line of 2^m elements.  A thread
-// holds NTT_R >> RR groups of 2^RR elements, group g = tg + s * (2^m / NTT_R)
+// holds 2^LR elements (LR levels per full round): 2^LR >> RR groups of 2^RR
+// elements, group g = tg + s * (2^m / 2^LR)
 // (consecutive threads take consecutive groups, so round 0 reads and the
 // last inverse round writes coalesced); group g spans
 // node*2^{m-d0} + off + j*2^{m-d0-RR}.
-template <int RR>
+template <int LR, int RR>
 __device__ __forceinline__ uint32_t group_base(int m, int d0, uint32_t tg, int s, uint32_t& node) {
-  const uint32_t g = tg + (uint32_t(s) << (m - NTT_LOGR));
+  const uint32_t g = tg + (uint32_t(s) << (m - LR));
   const int lst = m - d0 - RR;
   node = g >> lst;
   const uint32_t off = g & ((1u << lst) - 1u);
   return (node << (m - d0)) + off;
 }
 
-// Move the thread's 16 elements of round (d0, RR) between registers and a store.
-template <int RR, class Ld>
-__device__ __forceinline__ void round_load(uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, Ld&& ld) {
+// Move the thread's 2^LR elements of round (d0, RR) between registers and a store.
+template <int LR, int RR, class Ld>
+__device__ __forceinline__ void round_load(uint32_t (&x)[1 << LR], int m, int d0, uint32_t tg, Ld&& ld) {
 #pragma unroll
-  for (int s = 0; s < (NTT_R >> RR); ++s) {
+  for (int s = 0; s < ((1 << LR) >> RR); ++s) {
     uint32_t node;
-    const uint32_t b = group_base<RR>(m, d0, tg, s, node);
+    const uint32_t b = group_base<LR, RR>(m, d0, tg, s, node);
     const int lst = m - d0 - RR;
 #pragma unroll
     for (int j = 0; j < (1 << RR); ++j) x[s * (1 << RR) + j] = ld(b + (uint32_t(j) << lst));
   }
 }
 
-template <int RR, class St>
-__device__ __forceinline__ void round_store(const uint32_t (&x)[NTT_R], int m, int d0, uint32_t tg, St&& st) {
+template <int LR, int RR, class St>
+__device__ __forceinline__ void round_store(const uint32_t (&x)[1 << LR], int m, int d0, uint32_t tg, St&& st) {
 #pragma unroll
-  for (int s = 0; s < (NTT_R >> RR); ++s) {
+  for (int s = 0; s < ((1 << LR) >> RR); ++s) {
     uint32_t node;
-    const uint32_t b = group_base<RR>(m, d0, tg, s, node);
+    const uint32_t b = group_base<LR, RR>(m, d0, tg, s, node);
     const int lst = m - d0 - RR;
 #pragma unroll
     for (int j = 0; j < (1 << RR); ++j) st(b + (uint32_t(j) << lst), x[s * (1 << RR) + j]);
@@ -141,108 +142,111 @@ __device__ __forceinline__ void round_store(const uint32_t (&x)[NTT_R], int m, i
 
 // Load the roots of round (d0, RR) of the line whose root node is lineRoot
 // (heap index at level 0 of this pass).
-template <int RR>
-__device__ __forceinline__ void round_tw(RoundTw<RR>& T, int m, int d0, uint32_t tg, uint32_t lineRoot,
+template <int LR, int RR>
+__device__ __forceinline__ void round_tw(RoundTw<LR, RR>& T, int m, int d0, uint32_t tg, uint32_t lineRoot,
                                          const TwDev* tw) {
 #pragma unroll
-  for (int s = 0; s < RoundTw<RR>::G; ++s) {
+  for (int s = 0; s < RoundTw<LR, RR>::G; ++s) {
     uint32_t node;
-    group_base<RR>(m, d0, tg, s, node);
+    group_base<LR, RR>(m, d0, tg, s, node);
     const uint32_t H0 = (lineRoot << d0) + node;
 #pragma unroll
     for (int e = 0; e < RR; ++e)
 #pragma unroll
-      for (int blk = 0; blk < (1 << e); ++blk) T.w[s * RoundTw<RR>::W + (1 << e) - 1 + blk] = tw[(H0 << e) + blk];
+      for (int blk = 0; blk < (1 << e); ++blk) T.w[s * RoundTw<LR, RR>::W + (1 << e) - 1 + blk] = tw[(H0 << e) + blk];
   }
 }
 
-template <int RR>
-__device__ __forceinline__ void round_fwd(uint32_t (&x)[NTT_R], const RoundTw<RR>& T, uint32_t p, uint32_t q) {
+template <int LR, int RR>
+__device__ __forceinline__ void round_fwd(uint32_t (&x)[1 << LR], const RoundTw<LR, RR>& T, uint32_t p, uint32_t q) {
 #pragma unroll
-  for (int s = 0; s < RoundTw<RR>::G; ++s) group_fwd<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<RR>::W], p, q);
+  for (int s = 0; s < RoundTw<LR, RR>::G; ++s)
+    group_fwd<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<LR, RR>::W], p, q);
 }
 
-template <int RR>
-__device__ __forceinline__ void round_inv(uint32_t (&x)[NTT_R], const RoundTw<RR>& T, uint32_t p, uint32_t q) {
+template <int LR, int RR>
+__device__ __forceinline__ void round_inv(uint32_t (&x)[1 << LR], const RoundTw<LR, RR>& T, uint32_t p, uint32_t q) {
 #pragma unroll
-  for (int s = 0; s < RoundTw<RR>::G; ++s) group_inv<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<RR>::W], p, q);
+  for (int s = 0; s < RoundTw<LR, RR>::G; ++s)
+    group_inv<RR>(&x[s * (1 << RR)], &T.w[s * RoundTw<LR, RR>::W], p, q);
 }
 
-// Round schedule of a line of 2^M elements: rounds of four levels, the last
-// one taking the remainder.  Everything is fixed at compile time, so each
-// round reads and writes a statically known memory (global or LDS).
-template <int M>
+// Round schedule of a line of 2^M elements: rounds of LR levels (2^LR
+// elements per thread), the last one taking the remainder.  Everything is
+// fixed at compile time, so each round reads and writes a statically known
+// memory (global or LDS).
+template <int M, int LR>
 struct Sched {
-  static constexpr int NR = (M + 3) / 4;
+  static constexpr int NR = (M + LR - 1) / LR;
   template <int I>
-  static constexpr int rr() { return (M - 4 * I) < 4 ? (M - 4 * I) : 4; }
+  static constexpr int rr() { return (M - LR * I) < LR ? (M - LR * I) : LR; }
 };
 
-template <int M, int I>
-using SchedTw = RoundTw<Sched<M>::template rr<I>()>;
+template <int M, int LR, int I>
+using SchedTw = RoundTw<LR, Sched<M, LR>::template rr<I>()>;
 
 // Forward rounds I.. of a line: round 0 loads through gld, later rounds
 // through LDS; every round but the last stores to LDS, and the last leaves its
 // elements in registers.  With AHEAD, round I's roots arrive in `pre` and each
 // round first issues the loads of the next round's roots (so their latency
 // overlaps a round of butterflies and the LDS exchange); otherwise each round
-// loads its own roots (from a table in LDS) and `pre` is unused.  `at_last`
+// loads its own roots (from LDS or global memory) and `pre` is unused.  `at_last`
 // runs at the start of the last round (the caller's prefetches for what
 // follows).  With PRE, round 0's elements are already in x.
-template <int M, int I, bool AHEAD, bool PRE, class GLd, class LLd, class LSt, class Hook>
-__device__ __forceinline__ void fwd_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
+template <int M, int LR, int I, bool AHEAD, bool PRE, class GLd, class LLd, class LSt, class Hook>
+__device__ __forceinline__ void fwd_rounds(uint32_t (&x)[1 << LR], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
                                            uint32_t p, uint32_t q, GLd& gld, LLd& lld, LSt& lst,
-                                           const SchedTw<M, I>& pre, Hook&& at_last) {
-  constexpr int RR = Sched<M>::template rr<I>();
-  SchedTw<M, I> own;
-  if constexpr (!AHEAD) round_tw(own, M, 4 * I, tg, lineRoot, tw);
-  const SchedTw<M, I>& cur = AHEAD ? pre : own;
-  if constexpr (I + 1 < Sched<M>::NR) {
-    SchedTw<M, I + 1> nxt;
-    if constexpr (AHEAD) round_tw(nxt, M, 4 * (I + 1), tg, lineRoot, tw);
+                                           const SchedTw<M, LR, I>& pre, Hook&& at_last) {
+  constexpr int RR = Sched<M, LR>::template rr<I>();
+  SchedTw<M, LR, I> own;
+  if constexpr (!AHEAD) round_tw(own, M, LR * I, tg, lineRoot, tw);
+  const SchedTw<M, LR, I>& cur = AHEAD ? pre : own;
+  if constexpr (I + 1 < Sched<M, LR>::NR) {
+    SchedTw<M, LR, I + 1> nxt;
+    if constexpr (AHEAD) round_tw(nxt, M, LR * (I + 1), tg, lineRoot, tw);
     if constexpr (I == 0) {
-      if constexpr (!PRE) round_load<RR>(x, M, 4 * I, tg, gld);
+      if constexpr (!PRE) round_load<LR, RR>(x, M, LR * I, tg, gld);
     } else {
-      round_load<RR>(x, M, 4 * I, tg, lld);
+      round_load<LR, RR>(x, M, LR * I, tg, lld);
     }
-    round_fwd<RR>(x, cur, p, q);
-    round_store<RR>(x, M, 4 * I, tg, lst);
+    round_fwd<LR, RR>(x, cur, p, q);
+    round_store<LR, RR>(x, M, LR * I, tg, lst);
     __syncthreads();
-    fwd_rounds<M, I + 1, AHEAD, PRE>(x, tg, lineRoot, tw, p, q, gld, lld, lst, nxt, at_last);
+    fwd_rounds<M, LR, I + 1, AHEAD, PRE>(x, tg, lineRoot, tw, p, q, gld, lld, lst, nxt, at_last);
   } else {
     at_last();
     if constexpr (I == 0) {
-      if constexpr (!PRE) round_load<RR>(x, M, 4 * I, tg, gld);
+      if constexpr (!PRE) round_load<LR, RR>(x, M, LR * I, tg, gld);
     } else {
-      round_load<RR>(x, M, 4 * I, tg, lld);
+      round_load<LR, RR>(x, M, LR * I, tg, lld);
     }
-    round_fwd<RR>(x, cur, p, q);
+    round_fwd<LR, RR>(x, cur, p, q);
   }
 }
 
 // Inverse rounds I down to 0, starting (I = NR - 1) from the registers of the
 // last forward round; round 0 stores through gst.  Roots as in fwd_rounds:
 // with AHEAD each round issues the loads of the next (lower) round's roots.
-template <int M, int I, bool AHEAD, class GSt, class LLd, class LSt>
-__device__ __forceinline__ void inv_rounds(uint32_t (&x)[NTT_R], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
+template <int M, int LR, int I, bool AHEAD, class GSt, class LLd, class LSt>
+__device__ __forceinline__ void inv_rounds(uint32_t (&x)[1 << LR], uint32_t tg, uint32_t lineRoot, const TwDev* tw,
                                            uint32_t p, uint32_t q, GSt& gst, LLd& lld, LSt& lst,
-                                           const SchedTw<M, I>& pre) {
-  constexpr int RR = Sched<M>::template rr<I>();
-  SchedTw<M, I> own;
-  if constexpr (!AHEAD) round_tw(own, M, 4 * I, tg, lineRoot, tw);
-  const SchedTw<M, I>& cur = AHEAD ? pre : own;
+                                           const SchedTw<M, LR, I>& pre) {
+  constexpr int RR = Sched<M, LR>::template rr<I>();
+  SchedTw<M, LR, I> own;
+  if constexpr (!AHEAD) round_tw(own, M, LR * I, tg, lineRoot, tw);
+  const SchedTw<M, LR, I>& cur = AHEAD ? pre : own;
   if constexpr (I > 0) {
-    SchedTw<M, I - 1> nxt;
-    if constexpr (AHEAD) round_tw(nxt, M, 4 * (I - 1), tg, lineRoot, tw);
-    if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
-    round_inv<RR>(x, cur, p, q);
-    round_store<RR>(x, M, 4 * I, tg, lst);
+    SchedTw<M, LR, I - 1> nxt;
+    if constexpr (AHEAD) round_tw(nxt, M, LR * (I - 1), tg, lineRoot, tw);
+    if constexpr (I + 1 < Sched<M, LR>::NR) round_load<LR, RR>(x, M, LR * I, tg, lld);
+    round_inv<LR, RR>(x, cur, p, q);
+    round_store<LR, RR>(x, M, LR * I, tg, lst);
     __syncthreads();
-    inv_rounds<M, I - 1, AHEAD>(x, tg, lineRoot, tw, p, q, gst, lld, lst, nxt);
+    inv_rounds<M, LR, I - 1, AHEAD>(x, tg, lineRoot, tw, p, q, gst, lld, lst, nxt);
   } else {
-    if constexpr (I + 1 < Sched<M>::NR) round_load<RR>(x, M, 4 * I, tg, lld);
-    round_inv<RR>(x, cur, p, q);
-    round_store<RR>(x, M, 0, tg, gst);
+    if constexpr (I + 1 < Sched<M, LR>::NR) round_load<LR, RR>(x, M, LR * I, tg, lld);
+    round_inv<LR, RR>(x, cur, p, q);
+    round_store<LR, RR>(x, M, 0, tg, gst);
   }
 }
 
@@ -254,8 +258,9 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 template <int M, int MODE>
 __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
+  constexpr int LR = NTT_LOGR;
   constexpr uint32_t S = 1u << M, Tl = S / NTT_R;
-  constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
+  constexpr int NR = Sched<M, LR>::NR, DL = LR * (NR - 1), RL = Sched<M, LR>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
   const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
@@ -282,16 +287,16 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
   // instead of four.
   constexpr bool AH = (M != 12);
   uint32_t x[NTT_R];
-  SchedTw<M, 0> t0;
+  SchedTw<M, LR, 0> t0;
   if constexpr (AH) round_tw(t0, M, 0, tg, lineRoot, twf);
   if constexpr (MODE == ROW_FWD) {
-    fwd_rounds<M, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
-    round_store<RL>(x, M, DL, tg, gst);
+    fwd_rounds<M, LR, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
+    round_store<LR, RL>(x, M, DL, tg, gst);
   } else {
     // With AH the last forward round also issues the loads of the first
     // inverse round's roots and of the pointwise operand (MULC / MULV);
     // otherwise the operand is loaded just before the product.
-    SchedTw<M, NR - 1> ti;
+    SchedTw<M, LR, NR - 1> ti;
     uint32_t opv[NTT_R];
     auto load_opv = [&] {
       if constexpr (MODE != ROW_SQR) {
@@ -299,13 +304,13 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
 #pragma unroll
         for (int s = 0; s < (NTT_R >> RL); ++s) {
           uint32_t node;
-          const uint32_t b = group_base<RL>(M, DL, tg, s, node);
+          const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
 #pragma unroll
           for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
         }
       }
     };
-    fwd_rounds<M, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
+    fwd_rounds<M, LR, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
       if constexpr (AH) {
         round_tw(ti, M, DL, tg, lineRoot, twi);
         load_opv();
@@ -325,32 +330,53 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
         v = mont_c(mont(v, opv[i], p, q), sc, scq, p);
       }
     }
-    inv_rounds<M, NR - 1, AH>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
+    inv_rounds<M, LR, NR - 1, AH>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
   }
 }
 
 // Column group of a column-pass workgroup.  Workgroups are dispatched to the
 // eight XCDs round-robin (XCD = workgroup id mod 8), and each XCD has its own
-// L2; a group of 16 columns covers half of a 128-byte line, so workgroups x
-// and x + 8 (same XCD, dispatched together) take the two halves of one line:
-// within each run of 16 ids, x -> 2 (x mod 8) + (x div 8 mod 2).
-__device__ __forceinline__ uint32_t col_group() {
+// L2; a group of C columns covers 4C bytes of a 128-byte line, so the G = 32 / C
+// workgroups x, x + 8, ..., x + 8 (G - 1) (same XCD, dispatched together) take
+// the G parts of one line: within each run of 8G ids, x -> G (x mod 8) +
+// (x div 8 mod G), a permutation of the run.
+__device__ __forceinline__ uint32_t col_group(uint32_t C) {
   const uint32_t x = blockIdx.x;
-  if (gridDim.x % 16u != 0u) return x;
-  return (x & ~15u) | ((x & 7u) << 1) | ((x >> 3) & 1u);
+  const uint32_t G = 32u / C;
+  if (G < 2u || gridDim.x % (8u * G) != 0u) return x;
+  return (x & ~(8u * G - 1u)) | ((x & 7u) * G) | ((x >> 3) & (G - 1u));
 }
 
 // Column passes: the first M levels on columns of stride n2 = n / 2^M; `cols`
 // consecutive columns per block, interleaved in LDS so that global accesses
-// of a wavefront touch consecutive words.  All columns use the same 2^M - 1
-// roots (heap entries 1 .. 2^M - 1 of the table), which the block stages in
-// LDS after the column data while the data loads are in flight.
-// Grid: (n2 / cols, K, jobs); dynamic LDS (cols + 1) * 2^M words.
+// of a wavefront touch consecutive words.
+//
+// Columns of up to 2^9 elements: 16 elements per thread, 16 columns per block;
+// all columns use the same 2^M - 1 roots (heap entries 1 .. 2^M - 1), which
+// the block stages in LDS after the column data while the data loads are in
+// flight.  Columns of 2^10 and 2^11: 32 elements per thread (rounds of five
+// levels, so 2^10 takes one LDS exchange) and 8 columns per block; a block
+// of 2^10 is 256 threads and 32 KiB of LDS, so five fit a CU and a one-job
+// pass of five planes of 2^21 (1,280 blocks) is resident in a single round
+// (16 elements per thread and 16 columns leave 640 blocks for 512 slots).
+// Their roots are read from the table in global memory (L2-resident: every
+// block of a plane reads the same 2^M - 1 words).
+// Grid: (n2 / cols, K, jobs); dynamic LDS cols * 2^M words (+ 2^M for staged
+// roots).
 template <int M>
-__global__ void __launch_bounds__(1024, 8) k_col_fwd(const NttArgs A) {
+struct ColGeo {
+  static constexpr int LR = (M >= 10) ? 5 : 4;
+  static constexpr bool LDS_ROOTS = (LR == 4);
+  static constexpr int THREADS = (M >= 10) ? (8 << (M - LR)) : 1024;
+  static constexpr int WAVES_PER_EU = (M == 10) ? 5 : ((M == 11) ? 4 : 8);
+};
+
+template <int M>
+__global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k_col_fwd(const NttArgs A) {
   extern __shared__ uint32_t lds[];
+  constexpr int LR = ColGeo<M>::LR;
   constexpr uint32_t S = 1u << M;
-  constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
+  constexpr int NR = Sched<M, LR>::NR, DL = LR * (NR - 1), RL = Sched<M, LR>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
   const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
@@ -362,31 +388,40 @@ __global__ void __launch_bounds__(1024, 8) k_col_fwd(const NttArgs A) {
   // Block-uniform base pointers and 32-bit per-thread offsets (line + i n2 <
   // 2^32), so global accesses take a scalar base and a 32-bit vector offset.
   const uint32_t n2 = uint32_t(n >> M);
-  const size_t base = so + size_t(k) * n + size_t(col_group()) * C;
+  const size_t base = so + size_t(k) * n + size_t(col_group(C)) * C;
   const TwDev* twf = J.fwd + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
-  uint32_t* roots = lds + C * S;
 
   auto gld = [&](uint32_t i) { return src[line + i * n2]; };
   auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
   auto lld = [&](uint32_t i) { return lds[i * C + line]; };
   auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
 
-  uint32_t x[NTT_R];
-  round_load<Sched<M>::template rr<0>()>(x, M, 0, tg, gld);
-  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twf[i];
-  __syncthreads();
-  SchedTw<M, 0> unused;
-  fwd_rounds<M, 0, false, true>(x, tg, 1u, roots, p, A.q[k], gld, lld, lst, unused, [] {});
-  round_store<RL>(x, M, DL, tg, gst);
+  uint32_t x[1 << LR];
+  SchedTw<M, LR, 0> t0;  // round 0's roots when they are loaded ahead
+  if constexpr (ColGeo<M>::LDS_ROOTS) {
+    uint32_t* roots = lds + C * S;
+    round_load<LR, Sched<M, LR>::template rr<0>()>(x, M, 0, tg, gld);
+    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twf[i];
+    __syncthreads();
+    fwd_rounds<M, LR, 0, false, true>(x, tg, 1u, roots, p, A.q[k], gld, lld, lst, t0, [] {});
+  } else {
+    // Roots one round ahead for columns of 2^10 (96 VGPRs, five waves per
+    // SIMD); columns of 2^11 would spill, so they load each round's roots.
+    constexpr bool AH = (M == 10);
+    if constexpr (AH) round_tw(t0, M, 0, tg, 1u, twf);
+    fwd_rounds<M, LR, 0, AH, false>(x, tg, 1u, twf, p, A.q[k], gld, lld, lst, t0, [] {});
+  }
+  round_store<LR, RL>(x, M, DL, tg, gst);
 }
 
 template <int M>
-__global__ void __launch_bounds__(1024, 8) k_col_inv(const NttArgs A) {
+__global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k_col_inv(const NttArgs A) {
   extern __shared__ uint32_t lds[];
+  constexpr int LR = ColGeo<M>::LR;
   constexpr uint32_t S = 1u << M;
-  constexpr int NR = Sched<M>::NR, DL = 4 * (NR - 1), RL = Sched<M>::template rr<NR - 1>();
+  constexpr int NR = Sched<M, LR>::NR, DL = LR * (NR - 1), RL = Sched<M, LR>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
   const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.z / nj);
@@ -395,26 +430,30 @@ __global__ void __launch_bounds__(1024, 8) k_col_inv(const NttArgs A) {
   const size_t n = size_t(1) << A.LN;
   const uint32_t C = uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
-  // Block-uniform base pointers and 32-bit per-thread offsets (line + i n2 <
-  // 2^32), so global accesses take a scalar base and a 32-bit vector offset.
   const uint32_t n2 = uint32_t(n >> M);
-  const size_t base = so + size_t(k) * n + size_t(col_group()) * C;
+  const size_t base = so + size_t(k) * n + size_t(col_group(C)) * C;
   const TwDev* twi = J.inv + size_t(k) * n;
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
-  uint32_t* roots = lds + C * S;
 
   auto gld = [&](uint32_t i) { return src[line + i * n2]; };
   auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
   auto lld = [&](uint32_t i) { return lds[i * C + line]; };
   auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
 
-  uint32_t x[NTT_R];
-  round_load<RL>(x, M, DL, tg, gld);
-  for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twi[i];
-  __syncthreads();
-  SchedTw<M, NR - 1> unused;
-  inv_rounds<M, NR - 1, false>(x, tg, 1u, roots, p, A.q[k], gst, lld, lst, unused);
+  uint32_t x[1 << LR];
+  SchedTw<M, LR, NR - 1> ti;  // the first inverse round's roots when loaded ahead
+  round_load<LR, RL>(x, M, DL, tg, gld);
+  if constexpr (ColGeo<M>::LDS_ROOTS) {
+    uint32_t* roots = lds + C * S;
+    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) roots[i] = twi[i];
+    __syncthreads();
+    inv_rounds<M, LR, NR - 1, false>(x, tg, 1u, roots, p, A.q[k], gst, lld, lst, ti);
+  } else {
+    constexpr bool AH = (M == 10);
+    if constexpr (AH) round_tw(ti, M, DL, tg, 1u, twi);
+    inv_rounds<M, LR, NR - 1, AH>(x, tg, 1u, twi, p, A.q[k], gst, lld, lst, ti);
+  }
 }
 
 // Elementwise Montgomery scaling x -> x * s * 2^-32 of K planes (used once at
