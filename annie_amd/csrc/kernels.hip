@@ -254,12 +254,21 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 
 // Row pass: the levels [m1, LN) of one contiguous row of 2^M elements per
 // line, then (by mode) the pointwise product and the inverse row levels.
-// Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / NTT_R threads.
+// 16 elements per thread (LR = 4 levels per register round; 32 elements per
+// thread at rows of 2^11 measured no faster).
+// Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / 16 threads.
+template <int M>
+struct RowGeo {
+  static constexpr int LR = NTT_LOGR;
+  static constexpr int THREADS = 512;
+  static constexpr int WAVES_PER_EU = 4;
+};
+
 template <int M, int MODE>
-__global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
+__global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
-  constexpr int LR = NTT_LOGR;
-  constexpr uint32_t S = 1u << M, Tl = S / NTT_R;
+  constexpr int LR = RowGeo<M>::LR;
+  constexpr uint32_t S = 1u << M, Tl = S >> LR;
   constexpr int NR = Sched<M, LR>::NR, DL = LR * (NR - 1), RL = Sched<M, LR>::template rr<NR - 1>();
   const uint32_t nj = uint32_t(A.njobs);
   const NttJob& J = A.job[blockIdx.z % nj];
@@ -286,7 +295,7 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
   // rounds), where the extra registers would leave three waves per SIMD
   // instead of four.
   constexpr bool AH = (M != 12);
-  uint32_t x[NTT_R];
+  uint32_t x[1 << LR];
   SchedTw<M, LR, 0> t0;
   if constexpr (AH) round_tw(t0, M, 0, tg, lineRoot, twf);
   if constexpr (MODE == ROW_FWD) {
@@ -297,12 +306,12 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
     // inverse round's roots and of the pointwise operand (MULC / MULV);
     // otherwise the operand is loaded just before the product.
     SchedTw<M, LR, NR - 1> ti;
-    uint32_t opv[NTT_R];
+    uint32_t opv[1 << LR];
     auto load_opv = [&] {
       if constexpr (MODE != ROW_SQR) {
         const uint32_t* op = J.opnd + base;
 #pragma unroll
-        for (int s = 0; s < (NTT_R >> RL); ++s) {
+        for (int s = 0; s < ((1 << LR) >> RL); ++s) {
           uint32_t node;
           const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
 #pragma unroll
@@ -320,7 +329,7 @@ __global__ void __launch_bounds__(512, 4) k_row(const NttArgs A) {
     // Pointwise product on the leaves held in registers.
     const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
 #pragma unroll
-    for (int i = 0; i < NTT_R; ++i) {
+    for (int i = 0; i < (1 << LR); ++i) {
       uint32_t& v = x[i];
       if constexpr (MODE == ROW_SQR) {
         v = mont_c(mont(v, v, p, q), sc, scq, p);
