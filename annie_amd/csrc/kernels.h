@@ -11,7 +11,8 @@ namespace annie {
 // on the fly with one multiply, which halves the table traffic).
 typedef uint32_t TwDev;
 
-// Elements per thread in the NTT passes (four levels per register round).
+// Elements per thread in the row passes and in columns of up to 2^9 (four
+// levels per register round; columns of 2^10 and 2^11 take 32 and five).
 constexpr int NTT_R = 16;
 constexpr int NTT_LOGR = 4;
 // Consecutive coefficients per thread in the carry kernel.

@@ -17,9 +17,10 @@
 // Lengths above 2^13 use two passes over HBM per direction: a column pass runs
 // the first m1 levels on lines of stride n2 = n / 2^m1, a row pass runs the
 // remaining levels on contiguous rows of n2 elements, and fuses the pointwise
-// product and the inverse row levels.  Each thread holds 32 elements in
-// registers (two or more independent groups) and runs four levels per LDS
-// round trip.
+// product and the inverse row levels.  A thread holds 2^LR elements in
+// registers and runs up to LR levels per LDS round trip: 16 elements and four
+// levels in the row passes and in columns of up to 2^9, 32 elements and five
+// levels in columns of 2^10 and 2^11.
 #include <type_traits>
 
 #include "kernels.h"
