@@ -633,8 +633,15 @@ static int setup_device(Ctx& c, int device, size_t slots) {
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   c.cur = c.stream;
   // ANNIE_SINGLE_STREAM=1 keeps both halves of a squaring on one stream.
+  // The two-stream squaring costs the host ~0.08-0.11 ms per squaring to
+  // enqueue (16 launches and 4 event calls, against ~0.04 ms for 12 launches
+  // on one stream) and gains only what the second stream fills of the GPU, so
+  // it is used from K n >= 2^22 words per register.  Measured ms/squaring,
+  // one vs two streams: K = 3 at 2^16 0.071 / 0.086, 2^17 0.079 / 0.092, 2^18
+  // 0.100 / 0.109, 2^19 0.133 / 0.134; K = 5 at 2^21 0.515 / 0.495.
   const char* single = std::getenv("ANNIE_SINGLE_STREAM");
   if (single && single[0] == '1') c.dual_stream = false;
+  if ((size_t(c.pl.K) << c.pl.LN) < (size_t(1) << 22)) c.dual_stream = false;
   if (slots == 1 && c.m1 > 0 && c.dual_stream) {
     HIPCHK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
