@@ -432,6 +432,13 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
   bool stopped = false;
   std::vector<Big*> vals;
   std::vector<std::unique_ptr<Big>> store;
+  // Per-tick operation lists, reused across ticks and classes (cleared, not
+  // reallocated, every tick).
+  std::vector<uint32_t> set1, sq, sqa;
+  std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
+  std::map<int, std::vector<uint32_t>> gt;
+  std::vector<uint8_t> handled;
+  Big none;
   auto finish = [&](Class& c, size_t s) {
     Job* j = c.slot[s];
     annie_batch_result& r = out[j->idx];
@@ -504,9 +511,12 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       Class& c = kv.second;
       if (c.active == 0) continue;
       any = true;
-      std::vector<uint32_t> set1, sq, sqa;
-      std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
-      std::map<int, std::vector<uint32_t>> gt;
+      set1.clear();
+      sq.clear();
+      sqa.clear();
+      for (auto& g : cp) g.second.clear();
+      for (auto& g : ml) g.second.clear();
+      for (auto& g : gt) g.second.clear();
       // Deferred Gerbicz-Li products of all tests are taken together every
       // FLUSH ticks (a test forces its own earlier if it must).
       const bool flush = (++c.ticks % flush_ticks) == 0;
@@ -527,14 +537,17 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
         }
       }
       if (!set1.empty()) c.eng->set1(set1);
-      for (auto& g : cp) c.eng->copy(g.first.first, g.first.second, g.second);
+      for (auto& g : cp)
+        if (!g.second.empty()) c.eng->copy(g.first.first, g.first.second, g.second);
       if (!sq.empty()) {
         c.eng->square_mul(sq, sqa);
         squarings += double(sq.size());
       }
-      for (auto& g : ml) c.eng->mul(g.first.first, g.first.second, g.second);
-      std::vector<uint8_t> handled(c.slots, 0);
+      for (auto& g : ml)
+        if (!g.second.empty()) c.eng->mul(g.first.first, g.first.second, g.second);
+      handled.assign(c.slots, 0);
       for (auto& g : gt) {
+        if (g.second.empty()) continue;
         store.clear();
         vals.clear();
         for (size_t i = 0; i < g.second.size(); ++i) {
@@ -548,7 +561,6 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
           if (c.slot[g.second[i]]->m->done()) finish(c, g.second[i]);
         }
       }
-      Big none;
       for (size_t s = 0; s < c.slots; ++s) {
         if (!c.slot[s] || handled[s]) continue;
         c.slot[s]->m->advance(none.v);
