@@ -393,7 +393,9 @@ __global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
-  const uint32_t C = uint32_t(A.cols);
+  // 8 columns per block at 32 elements per thread (a compile-time constant, so
+  // that the LDS swizzle below folds); the engine's choice otherwise.
+  const uint32_t C = (LR == 5) ? 8u : uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   // Block-uniform base pointers and 32-bit per-thread offsets (line + i n2 <
   // 2^32), so global accesses take a scalar base and a 32-bit vector offset.
@@ -405,8 +407,12 @@ __global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k
 
   auto gld = [&](uint32_t i) { return src[line + i * n2]; };
   auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
-  auto lld = [&](uint32_t i) { return lds[i * C + line]; };
-  auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
+  // Element i of column `line` sits in LDS row i at column line ^ ((i >> LR) &
+  // (C - 1)): a bijection within each row (C is a power of two), so no extra
+  // space; without it the lanes of a wave that share `line` in a round whose
+  // groups are 2^LR consecutive elements (i = 2^LR tg + j) all hit one bank.
+  auto lld = [&](uint32_t i) { return lds[i * C + (line ^ ((i >> LR) & (C - 1u)))]; };
+  auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + (line ^ ((i >> LR) & (C - 1u)))] = v; };
 
   uint32_t x[1 << LR];
   SchedTw<M, LR, 0> t0;  // round 0's roots when they are loaded ahead
@@ -438,7 +444,9 @@ __global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
-  const uint32_t C = uint32_t(A.cols);
+  // 8 columns per block at 32 elements per thread (a compile-time constant, so
+  // that the LDS swizzle below folds); the engine's choice otherwise.
+  const uint32_t C = (LR == 5) ? 8u : uint32_t(A.cols);
   const uint32_t line = threadIdx.x % C, tg = threadIdx.x / C;
   const uint32_t n2 = uint32_t(n >> M);
   const size_t base = so + size_t(k) * n + size_t(col_group(C)) * C;
@@ -448,8 +456,12 @@ __global__ void __launch_bounds__(ColGeo<M>::THREADS, ColGeo<M>::WAVES_PER_EU) k
 
   auto gld = [&](uint32_t i) { return src[line + i * n2]; };
   auto gst = [&](uint32_t i, uint32_t v) { dst[line + i * n2] = v; };
-  auto lld = [&](uint32_t i) { return lds[i * C + line]; };
-  auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + line] = v; };
+  // Element i of column `line` sits in LDS row i at column line ^ ((i >> LR) &
+  // (C - 1)): a bijection within each row (C is a power of two), so no extra
+  // space; without it the lanes of a wave that share `line` in a round whose
+  // groups are 2^LR consecutive elements (i = 2^LR tg + j) all hit one bank.
+  auto lld = [&](uint32_t i) { return lds[i * C + (line ^ ((i >> LR) & (C - 1u)))]; };
+  auto lst = [&](uint32_t i, uint32_t v) { lds[i * C + (line ^ ((i >> LR) & (C - 1u)))] = v; };
 
   uint32_t x[1 << LR];
   SchedTw<M, LR, NR - 1> ti;  // the first inverse round's roots when loaded ahead
