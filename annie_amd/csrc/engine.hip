@@ -260,6 +260,9 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   if (C < 1) C = 1;
   if (C > 16) C = 16;
   if (C > n2) C = n2;
+  // The 32-element column kernels take 8 columns per block as a compile-time
+  // constant (n2 >= 2^10 whenever m1 >= 10, so C is never clipped there).
+  if (wide && C != 8u) return fail(ANNIE_ERR_SIZE, "column pass geometry");
   A.cols = int(C);
   const dim3 grid(n2 / C, c.pl.K, nj * c.nslots), block(C * Tl);
   const size_t shm = size_t(wide ? C : C + 1) * S * sizeof(uint32_t);
