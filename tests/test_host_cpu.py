@@ -116,3 +116,18 @@ def test_cli_expression_front_end(tmp_path):
     assert "3628801 is" in out.stdout
     out = subprocess.run([exe, "-q", "unknown(3)"], capture_output=True, text=True, timeout=60, cwd=w)
     assert out.returncode == 1 and "unknown function" in out.stderr
+
+
+def test_bench_reads_committed_traffic_for_every_profiled_kernel():
+    """bench.py takes the roofline `traffic` of the dominant kernel from the
+    committed rocprofv3 PMC summary: every kernel of the 40M-digit squaring
+    must have a parsable line there (HBM bytes at least its algorithmic bytes,
+    a VALU utilisation in (0, 1])."""
+    import bench
+    U = 5 * 2 ** 21 * 4
+    for kind in ("k_row<11,mulc>x1", "k_row<11,sqr>x2", "k_col_fwd<10>x1", "k_col_fwd<10>x2", "k_col_inv<10>x1",
+                 "k_col_inv<10>x2", "k_carry<5>x1", "k_carry<5>x2"):
+        traffic, valu = bench.measured_traffic(kind)
+        assert traffic is not None, kind
+        assert traffic >= 0.95 * bench.alg_bytes(kind, U), kind
+        assert valu is not None and 0.0 < valu <= 1.0, kind
