@@ -83,6 +83,7 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_twiddles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, u32p, u32p, sz]
     lib.annie_hip_batch_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, ctypes.c_int,
                                            ctypes.POINTER(h)]
+    lib.annie_hip_batch_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     lib.annie_hip_batch_load.argtypes = [h, sz, u64p, sz, ctypes.c_int32, u64p, sz, ctypes.POINTER(sz)]
     lib.annie_hip_batch_set1.argtypes = [h, u32p, sz]
     lib.annie_hip_batch_copy.argtypes = [h, sz, sz, u32p, sz]
@@ -122,6 +123,15 @@ def plan_constants(LN, K, pck, lbase):
     v = list(out)
     return {"p": v[0:5], "q": v[5:10], "sq_scale": v[10:15], "c32": v[15:20], "c64": v[20:25],
             "ginv": [v[25 + 5 * i:30 + 5 * i] for i in range(5)]}
+
+
+def batch_bytes(K, LN):
+    """(bytes per slot, bytes per batch) of device memory for a batch of plan (K, LN)."""
+    lib = load_library()
+    per, fixed = ctypes.c_size_t(), ctypes.c_size_t()
+    if lib.annie_hip_batch_bytes(K, LN, ctypes.byref(per), ctypes.byref(fixed)) != ANNIE_OK:
+        raise RuntimeError(_err(lib))
+    return per.value, fixed.value
 
 
 def twiddles(LN, k, table):
@@ -276,7 +286,7 @@ class CheckOpts(ctypes.Structure):
     _fields_ = [("is_euler", ctypes.c_int), ("a_ref", ctypes.c_int32), ("device", ctypes.c_int),
                 ("gmp_threshold_digits", ctypes.c_int), ("min_ln", ctypes.c_int), ("workdir", ctypes.c_char_p),
                 ("checkpoint_seconds", ctypes.c_double), ("quit_after", ctypes.c_long),
-                ("corrupt_at", ctypes.c_long), ("verbose", ctypes.c_int)]
+                ("corrupt_at", ctypes.c_long), ("verbose", ctypes.c_int), ("expression", ctypes.c_char_p)]
 
 
 class CheckResult(ctypes.Structure):
@@ -317,7 +327,7 @@ def check(N, is_euler=False, a=0, device=0, gmp_threshold_digits=10000, min_ln=1
     arr, n = int_to_limbs(N)
     o = CheckOpts(1 if is_euler else 0, a, device, gmp_threshold_digits, min_ln,
                   workdir.encode() if workdir else None, checkpoint_seconds, quit_after, corrupt_at,
-                  1 if verbose else 0)
+                  1 if verbose else 0, None)
     r = CheckResult()
     lib.annie_check(arr, n, ctypes.byref(o), ctypes.byref(r))
     return {"status": STATUS.get(r.status, r.status), "is_prp": bool(r.is_prp), "res64": r.res64,

@@ -1141,6 +1141,16 @@ int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int 
   return ANNIE_OK;
 }
 
+int annie_hip_batch_bytes(int K, int LN, size_t* per_slot, size_t* fixed) {
+  if (K < 2 || K > KMAX || LN < 4 || LN > 23) return fail(ANNIE_ERR_ARG, "bad plan");
+  const size_t n = size_t(1) << LN;
+  // setup_device's slot stride, the staging ring's two words per slot per
+  // entry and the graph buffers' two words per slot.
+  if (per_slot) *per_slot = (size_t(NREG) * size_t(K) * n + 2 * size_t(DIGIT_LIMBS) * n) * 4 + NSTAGE * 2 * 4 + 2 * 4;
+  if (fixed) *fixed = 4 * size_t(K) * n * sizeof(TwDev);
+  return ANNIE_OK;
+}
+
 int annie_hip_batch_load(annie_hip_t* h, size_t slot, const uint64_t* N, size_t nlimbs, int32_t a, uint64_t* factor,
                          size_t cap, size_t* nout) {
   if (!h || !N || nlimbs == 0 || a < 1) return fail(ANNIE_ERR_ARG, "bad argument");

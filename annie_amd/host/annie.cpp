@@ -150,6 +150,7 @@ static int check(const mpz_t n, const annie_check_opts& o, annie_check_result& r
     s = s.c_str();
     ns = (s.size() <= 40) ? s : s.substr(0, 8) + "[" + std::to_string(s.size() - 16) + " other digits]" +
                                      s.substr(s.size() - 8);
+    if (o.expression && std::strlen(o.expression) <= 40) ns = o.expression;
   }
 
   if (nsize <= 15) {  // src/annie.h:222-237

@@ -410,14 +410,15 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
                      [](const Job* x, const Job* y) { return x->arrival < y->arrival; });
 
   // Slots per class: every test of the class, capped by max_slots and by an
-  // equal share of the HBM budget (slot = 12 registers of K n words + digit
-  // buffers; each class also holds 4 twiddle tables).
+  // equal share of the HBM budget (the engine's bytes per slot and per batch).
   const double budget = o.hbm_budget_mb > 0 ? o.hbm_budget_mb * 1048576.0 : 64.0 * 1073741824.0;
   const size_t ncls = std::max<size_t>(classes.size(), 1);
   for (auto& kv : classes) {
     Class& c = kv.second;
-    const double K = std::get<0>(c.key), n = double(size_t(1) << std::get<2>(c.key));
-    const double slot_bytes = (12 * K + 6) * n * 4, fixed = 16 * K * n;
+    size_t per_slot = 0, fixed_bytes = 0;
+    if (annie_hip_batch_bytes(std::get<0>(c.key), std::get<2>(c.key), &per_slot, &fixed_bytes) != ANNIE_OK)
+      throw std::runtime_error(annie_hip_last_error());
+    const double slot_bytes = double(per_slot), fixed = double(fixed_bytes);
     size_t s = c.remaining;
     if (o.max_slots > 0) s = std::min(s, o.max_slots);
     const double share = budget / double(ncls) - fixed;

@@ -64,8 +64,11 @@ class Parser {
       mpz_clear(t);
     }
   }
+  // Precedence as in PARI/GP (the reference's evaluator, src/main.cpp:252):
+  // unary minus binds looser than '^' (-2^2 = -4) and tighter than '*'
+  // (2*-3 = -6); '^' is right associative and takes a signed exponent.
   void term(mpz_t r) {
-    power(r);
+    unary(r);
     for (;;) {
       mpz_t t;
       char op;
@@ -74,22 +77,12 @@ class Parser {
       else if (eat('%')) op = '%';
       else return;
       mpz_init(t);
-      power(t);
+      unary(t);
       if (op == '*') mpz_mul(r, r, t);
       else if (mpz_sgn(t) == 0) error("division by zero");
       else if (op == '/') mpz_tdiv_q(r, r, t);
       else mpz_mod(r, r, t);
       mpz_clear(t);
-    }
-  }
-  void power(mpz_t r) {
-    unary(r);
-    if (eat('^')) {
-      mpz_t e;
-      mpz_init(e);
-      power(e);  // right associative
-      mpz_pow_ui(r, r, small(e));
-      mpz_clear(e);
     }
   }
   void unary(mpz_t r) {
@@ -102,7 +95,18 @@ class Parser {
       unary(r);
       return;
     }
+    power(r);
+  }
+  void power(mpz_t r) {
     postfix(r);
+    if (eat('^')) {
+      mpz_t e;
+      mpz_init(e);
+      unary(e);  // right associative: 2^3^2 = 2^9
+      if (mpz_sgn(e) < 0) error("negative exponent (not an integer)");
+      mpz_pow_ui(r, r, small(e));
+      mpz_clear(e);
+    }
   }
   void postfix(mpz_t r) {
     primary(r);
@@ -274,6 +278,7 @@ int main(int argc, char* argv[]) {
     o.quit_after = -1;
     o.corrupt_at = -1;
     o.verbose = 1;
+    o.expression = expression.c_str();
     annie_check_result r{};
     annie_check(l.data(), l.size(), &o, &r);
     if (r.status == 1) std::printf("Validation failed!\n");

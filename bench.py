@@ -11,13 +11,19 @@ them.  Registers, twiddles and constants are resident in HBM before timing.
 Prints ONE JSON line.  `value` is ms per squaring (lower is better), timed
 with HIP events on the engine's stream over `--steps` squarings after
 `--warmup` untimed ones (the negacyclic half of each squaring runs on a
-second stream).  `roofline` is computed live for the kernel that takes the
-most time per squaring (HIP event pairs around its launches over a profile
-window of the same workload, issued on one stream so each kernel is timed
-alone); `cpu_baseline` is the reference's own
-CPU transform (src/transformCPU.h, built from the reference sources into
-oracle/_ref) timed on a bounded sample on one host core, run concurrently
-while the GPU context initialises.
+second stream), after a GPU clock warm-up of `--gpu-warmup-seconds`.  The
+residue is checked against a GMP fixture (tests/golden/cfg3_prefix.json,
+mpz_powm): runs of at most 1,000 squarings in all (lead-in, warm-up, timed and
+profile windows) end exactly at the fixture's 1,000-iteration prefix and
+compare the final residue; longer runs compare the residue after its
+100-iteration prefix, before the warm-up.  A mismatch prints an error line
+and exits 1.  `roofline` is computed live for the kernel that takes the most
+time per squaring (HIP event pairs around its launches over a profile window
+of the same workload, issued on one stream so each kernel is timed alone),
+against HBM and, with the VALU instruction count of the committed counter
+summary, against the VALU peak; `cpu_baseline` is the reference's own CPU
+transform (src/transformCPU.h, built from the reference sources into
+oracle/_ref) timed on a bounded sample on one host core after the GPU work.
 """
 import argparse
 import json
@@ -83,24 +89,76 @@ def alg_bytes(kind, U, K=5):
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "square_mul_40M_digits_pmc.md")
+FIXTURE = os.path.join(ROOT, "tests", "golden", "cfg3_prefix.json")
+# VALU peak: 256 CUs x 4 SIMDs x 16 lanes per cycle at 2.4 GHz (32-bit integer
+# multiply-high included: it issues at full rate on gfx950).
+VALU_PEAK = 256 * 64 * 2.4e9
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` and its VALU utilisation from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
-    (counters scaled by a plain copy of known size), or (None, None)."""
+def measured_counters(kernel):
+    """Counters of one launch of `kernel` from the committed rocprofv3 summary
+    of this workload: HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by a plain copy
+    of known size), VALU utilisation and VALU wave instructions per launch, or
+    {} when the summary has no line for it."""
     try:
         lines = open(PMC_SUMMARY).read().splitlines()
     except OSError:
-        return None, None
+        return {}
     for ln in lines:
         f = ln.split()
         if len(f) < 2 or f[0] != "traffic" or f[1] != kernel:
             continue
         kv = dict(x.split("=", 1) for x in f[2:])
         b = (float(kv["fetch_kib"]) * float(kv["fetch_factor"]) + float(kv["write_kib"]) * float(kv["write_factor"])) * 1024
-        return int(round(b)), float(kv["valu_util"]) if "valu_util" in kv else None
-    return None, None
+        out = {"traffic": int(round(b))}
+        if "valu_util" in kv:
+            out["valu_util"] = float(kv["valu_util"])
+        if "valu_insts" in kv:
+            out["valu_insts"] = float(kv["valu_insts"])
+        return out
+    return {}
+
+
+def load_fixture(candidate):
+    """{iterations: (res64, sha256)} of the GMP prefix fixture for the seeded
+    40M-digit N (tests/golden/cfg3_prefix.json), or {} for another N."""
+    try:
+        fx = json.load(open(FIXTURE))
+    except OSError:
+        return {}
+    if (fx["bits"], fx["seed"], fx["base"], fx["candidate"]) != (BITS, SEED, BASE, candidate):
+        return {}
+    return {e["iterations"]: (e["res64"], e["sha256"]) for e in fx["entries"]}
+
+
+def verify(T, N, iterations, fixture):
+    """Canonical residue of r0 after `iterations` loop iterations against the
+    GMP fixture (SHA-256 of the whole residue and its low 64 bits)."""
+    import hashlib
+    v = T.get(0)
+    res64, sha = fixture[iterations]
+    digest = hashlib.sha256(v.to_bytes((N.bit_length() + 7) // 8, "little")).hexdigest()
+    ok = (v & ((1 << 64) - 1)) == res64 and digest == sha
+    progress(f"residue after {iterations} iterations {'matches' if ok else 'DIFFERS FROM'} the GMP fixture")
+    return {"iterations": iterations, "res64": f"{v & ((1 << 64) - 1):016X}", "match": ok,
+            "against": "mpz_powm, " + os.path.relpath(FIXTURE, ROOT)}
+
+
+def host_info():
+    """Host cores and CPU model of the machine the CPU baseline ran on."""
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"host_cores": os.cpu_count(), "host_cores_usable": usable, "cpu_model": model}
 
 
 def progress(msg):
@@ -333,7 +391,9 @@ def bench_serve(args):
 def bench_proth3m(args):
     """ms/squaring of the Euler (Proth) test of k*2^3,000,000+1 (seeded 32-bit
     odd k), the reference's plan K = 4, LN = 16; the reference CPU transform
-    timed on a bounded sample of the same N on one core."""
+    timed on a bounded sample of the same N on one core.  After the timed
+    squarings the run continues (untimed) to the next iteration count of the
+    GMP fixture tests/golden/cfg2_proth.json and compares the residue."""
     import annie_amd
     import oracle  # the CPU baseline leg only
     rng = random.Random(2)
@@ -347,13 +407,31 @@ def bench_proth3m(args):
         T.close()
     e = (N - 1) >> 1
     total = args.warmup + args.steps
-    bits = [int(c) for c in format(e >> (e.bit_length() - total), "b")]
+    fx_path = os.path.join(ROOT, "tests", "golden", "cfg2_proth.json")
+    check_at = None
+    if os.path.exists(fx_path):
+        fx = json.load(open(fx_path))
+        if fx["k"] == k:
+            # iteration bits(k) + j holds a^(k 2^j)
+            pts = sorted((k.bit_length() + p["j"], p["low64"]) for p in fx["prefix"])
+            check_at = next((p for p in pts if p[0] >= total), None)
+    nbits = max(total, check_at[0] if check_at else 0)
+    bits = [int(c) for c in format(e >> (e.bit_length() - nbits), "b")]
     T.set1()
     T.square_mul_bits(bits[:args.warmup], a)
     T.sync()
-    ms = T.time_square_mul_bits(bits[args.warmup:], a) / args.steps
+    ms = T.time_square_mul_bits(bits[args.warmup:total], a) / args.steps
     plan = (T.K, T.pck, T.LN, T.lbase)
+    verified = []
+    if check_at:
+        T.square_mul_bits(bits[total:check_at[0]], a)
+        low = T.get(0) & ((1 << 64) - 1)
+        verified.append({"iterations": check_at[0], "res64": f"{low:016X}", "match": low == check_at[1],
+                         "against": "GMP, " + os.path.relpath(fx_path, ROOT)})
     T.close()
+    if any(not v["match"] for v in verified):
+        print(json.dumps({"error": "GPU residue differs from the GMP fixture", "verified": verified}), flush=True)
+        sys.exit(1)
     progress(f"GPU {ms:.4f} ms/squaring; CPU baseline (reference transformCPU, one core)")
     cpu = None
     if not args.no_cpu_baseline:
@@ -366,7 +444,7 @@ def bench_proth3m(args):
             done += 1
         dt = time.time() - t0
         cpu = {"value": round(dt * 1e3 / done, 3), "unit": "ms/squaring", "cores": 1, "kind": kind,
-               "sample": f"{done} squarings of the same N ({dt:.1f} s), one core"}
+               "sample": f"{done} squarings of the same N ({dt:.1f} s), one core", **host_info()}
         C.close()
     return {
         "metric": "ms/squaring, Proth k*2^3,000,000+1 (Euler test)", "value": round(ms, 5),
@@ -374,6 +452,7 @@ def bench_proth3m(args):
         "higher_is_better": False, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"Euler test of k*2^3000000+1, k = {k} (seed 2), base {a}",
                    "K": plan[0], "pck": plan[1], "LN": plan[2], "lbase": plan[3]},
+        "verified": verified,
         "cpu_baseline": cpu,
     }
 
@@ -381,9 +460,10 @@ def bench_proth3m(args):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--steps", type=int, default=800)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--gpu-warmup-seconds", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", type=int, default=None, help=argparse.SUPPRESS)
@@ -425,64 +505,115 @@ def main():
         N = workload_N(skip=skip)
         progress(f"candidate {skip}: {N.bit_length()}-bit N; creating the context (N^-1 mod R takes about a minute)")
         t0 = time.time()
-        cpu = None
-        if rank == 0 and not args.no_cpu_baseline:
-            cpu = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", str(skip),
-                                    "--cpu-seconds", str(args.cpu_seconds)],
-                                   stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
         T = annie_amd.Transform(N, BASE)
         init_s = time.time() - t0
         progress(f"context ready in {init_s:.1f} s (init_failed={T.init_failed()})")
         if not T.init_failed():
             break
-        if cpu is not None:
-            cpu.kill()
-            cpu.wait()
         T.close()
         skip += 1
 
     U = T.K * T.n * 4
+    fixture = load_fixture(skip)
     total = args.warmup + args.steps + args.profile_steps
-    bits = exponent_bits(N, total)
+    # Residue check against the GMP fixture: when the whole run fits in the
+    # fixture's longest prefix, untimed lead-in squarings bring the run to
+    # exactly that many iterations and the final residue is compared; longer
+    # runs compare the residue after the fixture's shortest prefix, taken
+    # before the warm-up.
+    longest = max(fixture) if fixture else 0
+    if fixture and total <= longest:
+        lead, pre = longest - total, 0
+    else:
+        lead, pre = 0, (min(fixture) if fixture else 0)
+    bits = exponent_bits(N, pre + lead + total)
+    verified = []
     T.set1()
-    progress(f"warmup: {args.warmup} squarings")
-    T.square_mul_bits(bits[:args.warmup], BASE)
+    if pre:
+        progress(f"residue check: {pre} squarings against the GMP fixture")
+        T.square_mul_bits(bits[:pre], BASE)
+        verified.append(verify(T, N, pre, fixture))
+    pos = pre
+    if lead:
+        progress(f"lead-in: {lead} untimed squarings")
+        T.square_mul_bits(bits[pos:pos + lead], BASE)
+        pos += lead
+    # Clock warm-up: products on register 2 (r0 and r1 untouched) until the
+    # GPU has been busy for --gpu-warmup-seconds.
+    T.copy(2, 0)
+    w0 = time.time()
+    while time.time() - w0 < args.gpu_warmup_seconds:
+        for _ in range(20):
+            T.mul(2, 2)
+        T.sync()
+    progress(f"warmup: {args.warmup} squarings (after {time.time() - w0:.1f} s of GPU clock warm-up)")
+    T.square_mul_bits(bits[pos:pos + args.warmup], BASE)
+    pos += args.warmup
     T.sync()
     progress(f"timed region: {args.steps} squarings")
     w0 = time.time()
-    ms = T.time_square_mul_bits(bits[args.warmup:args.warmup + args.steps], BASE)
+    ms = T.time_square_mul_bits(bits[pos:pos + args.steps], BASE)
     wall = time.time() - w0
+    pos += args.steps
     ms_per_sq = ms / args.steps
 
     progress(f"{ms_per_sq:.4f} ms/squaring; profile window: {args.profile_steps} squarings")
     T.profile_begin()
-    T.square_mul_bits(bits[args.warmup + args.steps:], BASE)
+    T.square_mul_bits(bits[pos:pos + args.profile_steps], BASE)
     prof = T.profile_end()
+    pos += args.profile_steps
+    if fixture and pos in fixture:
+        verified.append(verify(T, N, pos, fixture))
     per_sq = {k: tot / args.profile_steps for k, (cnt, tot) in prof.items()}
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, tot = prof[dom]
     avg_ms = tot / cnt
     achieved = alg_bytes(dom, U, T.K) / (avg_ms * 1e-3) / 1e9
     b_alg_sq = 27 * U  # (16P - 5) U with P = 2 two-pass directions at this length
-    traffic, valu_util = measured_traffic(dom)
+    pmc = measured_counters(dom)
 
     # A plain register copy of known size (U bytes read, U written) for
     # calibrating HBM byte counters in profiling runs.
     T.copy(1, 0)
     T.sync()
+    plan = (T.K, T.n, T.LN, T.lbase, T.pck)
+    T.close()
+
+    if any(not v["match"] for v in verified):
+        print(json.dumps({"error": "GPU residue differs from the GMP fixture", "verified": verified}), flush=True)
+        sys.exit(1)
 
     cpu_res = None
-    if cpu is not None:
-        progress("waiting for the CPU baseline (reference transformCPU on one core)")
-        while cpu.poll() is None:
-            time.sleep(10)
-            progress("CPU baseline still running")
-        out, err = cpu.communicate(timeout=900)
+    if rank == 0 and not args.no_cpu_baseline:
+        # After the GPU work, so that it competes with nothing that is timed.
+        progress("CPU baseline (reference transformCPU on one core)")
+        cpu = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", str(skip),
+                              "--cpu-seconds", str(args.cpu_seconds)], capture_output=True, text=True, timeout=900)
         try:
-            cpu_res = json.loads(out.strip().splitlines()[-1])
+            cpu_res = json.loads(cpu.stdout.strip().splitlines()[-1])
         except (ValueError, IndexError):
-            cpu_res = {"error": (err or out)[-300:]}
+            cpu_res = {"error": (cpu.stderr or cpu.stdout)[-300:]}
 
+    roof = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": pmc.get("traffic"),
+        "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if pmc else None,
+        "alg_bytes_per_launch": alg_bytes(dom, U, T.K),
+        "avg_launch_ms": round(avg_ms, 5),
+    }
+    if pmc.get("valu_insts"):
+        # VALU roof of the same launch: its counted VALU wave instructions
+        # (64 lanes each) over its live duration, against 256 CUs x 64 lanes
+        # per cycle at 2.4 GHz.
+        rate = pmc["valu_insts"] * 64 / (avg_ms * 1e-3)
+        roof["valu"] = {"achieved": round(rate / 1e12, 2), "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
+                        "frac": round(rate / VALU_PEAK, 4), "counter_util": pmc.get("valu_util")}
+        roof["limiter"] = "valu" if rate / VALU_PEAK > achieved / HBM_PEAK_GBS else "hbm"
     line = {
         "metric": "ms/squaring at ~40M digits",
         "value": round(ms_per_sq, 5),
@@ -498,29 +629,20 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "Fermat PRP square_mul loop, seeded random odd 132,900,000-bit N (~40M digits), base 3",
-            "bits": BITS, "seed": SEED, "candidate": skip, "K": T.K, "n": T.n, "LN": T.LN, "lbase": T.lbase,
-            "pck": T.pck, "parallelism": "single GPU",
+            "bits": BITS, "seed": SEED, "candidate": skip, "K": plan[0], "n": plan[1], "LN": plan[2],
+            "lbase": plan[3], "pck": plan[4], "parallelism": "single GPU",
         },
+        "verified": verified,
+        "gpu_warmup_s": args.gpu_warmup_seconds,
         "squarings_per_s": round(1e3 / ms_per_sq, 1),
         "wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
         "init_s": round(init_s, 1),
         "squaring_alg_bytes": b_alg_sq,
         "squaring_hbm_frac": round(b_alg_sq / (ms_per_sq * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernels_ms_per_squaring": {k: round(v, 5) for k, v in sorted(per_sq.items(), key=lambda kv: -kv[1])},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if traffic is not None else None,
-            "alg_bytes_per_launch": alg_bytes(dom, U, T.K),
-            "avg_launch_ms": round(avg_ms, 5),
-            "valu_util": valu_util,
-        },
+        "roofline": roof,
     }
+    host = host_info()
     if cpu_res is not None and "ms_per_squaring" in cpu_res:
         line["cpu_baseline"] = {
             "value": round(cpu_res["ms_per_squaring"], 3),
@@ -529,10 +651,10 @@ def main():
             "kind": cpu_res["kind"],
             "sample": f"{cpu_res['squarings']} squarings of the same N ({cpu_res['seconds']:.1f} s, after "
                       f"{cpu_res['init_s']:.0f} s init), {cpu_res['transform']}, one pinned host core",
+            **host,
         }
     elif cpu_res is not None:
-        line["cpu_baseline"] = {"value": None, "error": cpu_res.get("error")}
-    T.close()
+        line["cpu_baseline"] = {"value": None, "error": cpu_res.get("error"), **host}
     if rank == 0:
         print(json.dumps(line), flush=True)
 

@@ -123,6 +123,10 @@ int annie_hip_read_tq(annie_hip_t* ctx, uint32_t* out, size_t cap_words);
  * lists hold distinct slot ids that have a modulus loaded.  A batch is
  * released with annie_hip_destroy. */
 int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int device, annie_hip_t** out);
+/* Device memory of a batch of plan (K, LN): bytes per slot (its registers,
+ * digit buffers and slot-list staging) and bytes held once per batch (twiddle
+ * tables), for admission under an HBM budget. */
+int annie_hip_batch_bytes(int K, int LN, size_t* per_slot, size_t* fixed);
 /* Loads modulus N (base a) into a slot: its plan must be the batch's.  Returns
  * ANNIE_INIT_FAILED with gcd(N, R) in `factor` when N is not invertible mod R
  * (src/transformGPU.h:652-661). */

@@ -24,6 +24,8 @@ typedef struct {
   long quit_after;            /* >= 0: stop (checkpoint + Aborted) after that many loop iterations */
   long corrupt_at;            /* >= 0: corrupt Reg0 at that loop iteration (GL check must fail) */
   int verbose;                /* print progress / result lines */
+  const char* expression;     /* N as typed, named in messages when at most 40 characters
+                                 (src/annie.h:219); NULL = the decimal value */
 } annie_check_opts;
 
 typedef struct {

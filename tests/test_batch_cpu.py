@@ -93,3 +93,24 @@ def test_autotuner_picks_a_measured_config(tmp_path):
     assert autotune.load_cache(path)[autotune.key_of((4, 2, 10, 28))]["best"] == b1
     # cached: no re-measurement needed
     assert autotune.tuned((4, 2, 10, 28), tests, path=path, configs=[{"bogus": 1}], engine="gmp") == b1
+
+
+def test_hbm_budget_bounds_the_slots_of_each_class():
+    """Admission under an HBM budget: each class gets slots whose device
+    memory (the engine's own bytes per slot and per batch) fits its equal
+    share of the budget, and the tests beyond those slots queue and still
+    finish with the oracle's results."""
+    tests = [{"N": proth(k, 2000), "is_euler": True} for k in (3, 5, 9, 15, 17, 21, 27, 33)]
+    plans = {annie_amd.select_plan(t["N"], oracle.base(t["N"], True)) for t in tests}
+    assert len(plans) == 1
+    K, pck, LN, lbase = plans.pop()
+    per_slot, fixed = annie_amd.batch_bytes(K, LN)
+    # NREG registers of K n words plus 2 x 3 digit planes per slot, 4 twiddle tables per batch
+    assert per_slot >= (13 * K + 6) * (4 << LN) and fixed == 16 * K << LN
+    budget = fixed + 3 * per_slot + per_slot // 2          # room for exactly 3 slots
+    res, st = annie_amd.batch_run(tests, engine="gmp", hbm_budget_mb=budget / 1048576.0)
+    assert st["slots"] == 3
+    assert st["slots"] * per_slot + fixed <= budget
+    _assert_matches(tests, res)
+    admitted = sorted(r["admitted"] for r in res if r["status"] == "success")
+    assert admitted[-1] > admitted[2]  # later tests waited for a free slot

@@ -113,9 +113,32 @@ def test_cli_expression_front_end(tmp_path):
     out = subprocess.run([exe, "-q", "(2^127-1)*3+2*0", "-w", w], capture_output=True, text=True, timeout=120)
     assert "composite" in out.stdout
     out = subprocess.run([exe, "-q", "10!+1"], capture_output=True, text=True, timeout=60, cwd=w)
-    assert "3628801 is" in out.stdout
+    assert "10!+1 is" in out.stdout  # the expression as typed (src/annie.h:219)
     out = subprocess.run([exe, "-q", "unknown(3)"], capture_output=True, text=True, timeout=60, cwd=w)
     assert out.returncode == 1 and "unknown function" in out.stderr
+
+
+@pytest.mark.parametrize("expr,value", [("-2^2+11", 7), ("2*-3+13", 7), ("2^3^2-1", 511), ("-(2)^2+13", 9),
+                                        ("3!-(-1)", 7), ("10-2-1", 7), ("2^2^0+2", 4)])
+def test_cli_expression_precedence_as_pari(tmp_path, expr, value):
+    """Operator precedence of PARI/GP, the reference's evaluator: unary minus
+    below '^' (-2^2 = -4) and above '*', '^' right associative.  Small values
+    are decided without the loop (src/annie.h:222-236), so the verdict names
+    the value's primality (a wrong precedence gives 15, 17 or 3 in three of
+    the cases, which flips the verdict)."""
+    exe = os.path.join(annie_amd.LIBDIR, "annie")
+    out = subprocess.run([exe, "-q", expr], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert out.returncode == 0, out.stderr
+    prime = value > 1 and all(value % d for d in range(2, int(value ** 0.5) + 1))
+    assert f"{expr} is {'prime' if prime else 'composite'}." in out.stdout, out.stdout
+
+
+@pytest.mark.parametrize("expr", ["2^-1", "-2^2", "2^(0-3)"])
+def test_cli_rejects_non_positive_or_fractional(tmp_path, expr):
+    exe = os.path.join(annie_amd.LIBDIR, "annie")
+    out = subprocess.run([exe, "-q", expr], capture_output=True, text=True, timeout=60, cwd=str(tmp_path))
+    assert out.returncode == 1 and "Error" in out.stderr
+
 
 
 def test_bench_reads_committed_traffic_for_every_profiled_kernel():
@@ -127,7 +150,7 @@ def test_bench_reads_committed_traffic_for_every_profiled_kernel():
     U = 5 * 2 ** 21 * 4
     for kind in ("k_row<11,mulc>x1", "k_row<11,sqr>x2", "k_col_fwd<10>x1", "k_col_fwd<10>x2", "k_col_inv<10>x1",
                  "k_col_inv<10>x2", "k_carry<5>x1", "k_carry<5>x2"):
-        traffic, valu = bench.measured_traffic(kind)
-        assert traffic is not None, kind
-        assert traffic >= 0.95 * bench.alg_bytes(kind, U), kind
-        assert valu is not None and 0.0 < valu <= 1.0, kind
+        c = bench.measured_counters(kind)
+        assert c.get("traffic") is not None, kind
+        assert c["traffic"] >= 0.95 * bench.alg_bytes(kind, U), kind
+        assert c.get("valu_util") is not None and 0.0 < c["valu_util"] <= 1.0, kind
