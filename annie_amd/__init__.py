@@ -23,6 +23,10 @@ Reg0, Reg1, Reg2 = 0, 1, 2
 
 ANNIE_OK = 0
 ANNIE_INIT_FAILED = 1
+ANNIE_ERR = -1
+ANNIE_ERR_SIZE = -2
+ANNIE_ERR_ARG = -3
+ANNIE_ERR_NOMEM = -4
 
 _lib = None
 
@@ -89,6 +93,8 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_batch_copy.argtypes = [h, sz, sz, u32p, sz]
     lib.annie_hip_batch_square_mul.argtypes = [h, u32p, u32p, sz]
     lib.annie_hip_batch_mul.argtypes = [h, sz, sz, u32p, sz]
+    lib.annie_hip_batch_square_run.argtypes = [h, u32p, sz, u32p, sz]
+    lib.annie_hip_batch_run_supported.argtypes = [h]
     lib.annie_hip_batch_get.argtypes = [h, sz, u32p, sz, u64p, sz, ctypes.POINTER(sz)]
     lib.annie_hip_batch_sync.argtypes = [h]
     lib.annie_hip_batch_set_graphs.argtypes = [h, ctypes.c_int]
@@ -351,7 +357,8 @@ class BatchTest(ctypes.Structure):
 class BatchOpts(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int), ("device", ctypes.c_int), ("max_slots", ctypes.c_size_t),
                 ("hbm_budget_mb", ctypes.c_double), ("realtime", ctypes.c_int), ("time_limit", ctypes.c_double),
-                ("verbose", ctypes.c_int), ("graphs", ctypes.c_int), ("flush_ticks", ctypes.c_int)]
+                ("verbose", ctypes.c_int), ("graphs", ctypes.c_int), ("flush_ticks", ctypes.c_int),
+                ("run_steps", ctypes.c_int), ("run_min_active", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -367,13 +374,16 @@ class BatchStats(ctypes.Structure):
 
 
 def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
-              verbose=False, graphs=False, flush_ticks=0):
+              verbose=False, graphs=False, flush_ticks=0, run_steps=0, run_min_active=0):
     """Run many tests together, one engine batch per length class, with
     continuous admission (annie_batch_run in include/annie_host.h).
 
     `tests`: iterable of dicts {"N": int, "is_euler": bool, "a": int (Fermat
     base, 0 = 2), "arrival": seconds}.  engine "gpu" is the gfx950 engine;
     "gmp" runs the same driver on exact GMP transforms on the CPU.
+    run_steps > 0: classes with at least run_min_active running tests take up
+    to run_steps squarings per test in one launch (squaring runs, lengths
+    2^9 .. 2^13); results are the same either way.
     Returns (results, stats): one dict per test (status, is_prp, res64, gl_ok,
     a, iterations, arrival, admitted, finished, message) and a stats dict.
     """
@@ -390,7 +400,8 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
         keep.append(limbs)
         arr[i] = BatchTest(limbs, n, 1 if t.get("is_euler") else 0, int(t.get("a", 0)), float(t.get("arrival", 0.0)))
     opts = BatchOpts(0 if engine == "gpu" else 1, device, max_slots, hbm_budget_mb, 1 if realtime else 0,
-                     time_limit, 1 if verbose else 0, 1 if graphs else 0, int(flush_ticks))
+                     time_limit, 1 if verbose else 0, 1 if graphs else 0, int(flush_ticks), int(run_steps),
+                     int(run_min_active))
     res = (BatchResult * max(len(tests), 1))()
     st = BatchStats()
     rc = lib.annie_batch_run(arr, len(tests), ctypes.byref(opts), res, ctypes.byref(st))

@@ -36,6 +36,8 @@ template <int K>
 __global__ void k_digits_to_res(const uint32_t* digits, uint32_t* dst, const CarryArgs A);
 __global__ void k_copy_slots(uint32_t* base, size_t dst_off, size_t src_off, size_t len, const uint32_t* slot_ids,
                              size_t slot_stride);
+template <int M, int K, int QS>
+__global__ void k_sqr_run(const RunArgs R);
 
 static thread_local std::string g_last_error;
 
@@ -107,6 +109,7 @@ struct Ctx {
   const uint32_t* slot_ids = nullptr;
   int nslots = 1;
   Stage stage;
+  Stage run_stage;  // slot lists and multipliers of squaring runs (allocated on first use)
   std::vector<std::unique_ptr<Mpz>> slotN;  // modulus of each slot (batches)
   // Batched squaring replayed from captured HIP graphs: the slot list and
   // multipliers live at fixed device addresses (graph_buf: ids, then
@@ -360,28 +363,49 @@ static CarryJob cjob(const uint32_t* src, uint32_t* dst, uint32_t a, int nega, i
   return j;
 }
 
-// reduceMMGW (src/transformGPU.h:523-538): with T_R in RTR and T_Q (raw digits)
-// in tq(): m' = T_R N^-1 mod R, canonical, + 2; dst = (m' + 2) N - T_Q mod Q.
-// snap (batches): per-slot flags; a flagged slot also gets the result in RSNAP.
-// join: T_Q is being produced on the aux stream; wait for it before the last
-// carry.  The (m' + 2) N product works in RSQ, so RWQ may be in use on aux.
-static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr, bool join = false) {
+// Buffers of one squaring: the operand x, the result y (may equal x), and the
+// work registers: wr / wq the cyclic / negacyclic convolutions, tr T_R then
+// m' + 2, sq the (m' + 2) N product, tq the T_Q digits (DIGIT_LIMBS planes).
+struct SqBufs {
+  const uint32_t* x;
+  uint32_t* y;
+  uint32_t *wr, *wq, *tr, *sq, *tq;
+};
+
+// The context's own work registers, r0 in and out.
+static SqBufs ctx_bufs(const Ctx& c) {
+  return SqBufs{c.reg(R0), c.reg(R0), c.reg(RWR), c.reg(RWQ), c.reg(RTR), c.reg(RSQ), c.tq()};
+}
+
+// reduceMMGW (src/transformGPU.h:523-538): with T_R in B.tr and T_Q (raw
+// digits) in B.tq: m' = T_R N^-1 mod R, canonical, + 2; dst = (m' + 2) N - T_Q
+// mod Q.  snap (batches): per-slot flags; a flagged slot also gets the result
+// in RSNAP.  join: T_Q is being produced on the aux stream; wait for it before
+// the last carry.  The (m' + 2) N product works in B.sq, so B.wq may be in
+// use on aux.
+static int reduce(Ctx& c, const SqBufs& B, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr,
+                  bool join = false) {
   int rc;
-  NttJob j = job(c.reg(RTR), c.reg(RWR), c.fwdR(), c.invR(), c.reg(RNIR));
+  NttJob j = job(B.tr, B.wr, c.fwdR(), c.invR(), c.reg(RNIR));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
-  CarryJob cj = cjob(c.reg(RWR), c.reg(RTR), 1u, 0, 1, 0);
+  CarryJob cj = cjob(B.wr, B.tr, 1u, 0, 1, 0);
   if ((rc = launch_carry(c, &cj, 1))) return rc;
-  j = job(c.reg(RTR), c.reg(RSQ), c.fwdQ(), c.invQ(), c.reg(RNQ));
+  j = job(B.tr, B.sq, c.fwdQ(), c.invQ(), c.reg(RNQ));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
   if (join) HIPCHK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
-  cj = cjob(c.reg(RSQ), dst, 1u, 1, 0, raw_out ? 1 : 0, c.tq());
+  cj = cjob(B.sq, dst, 1u, 1, 0, raw_out ? 1 : 0, B.tq);
   if (snap != nullptr) cj.dst2 = c.reg(RSNAP);
   return launch_carry(c, &cj, 1, nullptr, snap);
 }
 
-// squareMMGW (src/transformGPU.h:556-568).  a_slot: per-slot multipliers of a
-// batch operation (device array in slot-list order), replacing a.
-static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
+static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr, bool join = false) {
+  return reduce(c, ctx_bufs(c), dst, raw_out, snap, join);
+}
+
+// squareMMGW (src/transformGPU.h:556-568): B.y = a B.x^2.  a_slot: per-slot
+// multipliers of a batch operation (device array in slot-list order),
+// replacing a.
+static int square_mul(Ctx& c, const SqBufs& B, uint32_t a, const uint32_t* a_slot = nullptr) {
   int rc;
   if (c.aux != nullptr && a_slot == nullptr && !c.profiling) {
     // Two streams (not while per-kernel timing is on, so that each kernel is
@@ -389,29 +413,33 @@ static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
     // the m' chain on the main stream, whose one-job passes leave much of the
     // GPU idle; it is forked after the cyclic half's carry (0.5% faster than
     // forking before the cyclic half) and the last carry joins them.
-    const NttJob jr = job(c.reg(R0), c.reg(RWR), c.fwdR(), c.invR());
+    const NttJob jr = job(B.x, B.wr, c.fwdR(), c.invR());
     if ((rc = convolve(c, ROW_SQR, &jr, 1))) return rc;
-    const CarryJob cr = cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0);
+    const CarryJob cr = cjob(B.wr, B.tr, a, 0, 0, 0);
     if ((rc = launch_carry(c, &cr, 1))) return rc;
     HIPCHK(hipEventRecord(c.ev_fork, c.stream));
     HIPCHK(hipStreamWaitEvent(c.aux, c.ev_fork, 0));
     c.cur = c.aux;
-    const NttJob jq = job(c.reg(R0), c.reg(RWQ), c.fwdQ(), c.invQ());
+    const NttJob jq = job(B.x, B.wq, c.fwdQ(), c.invQ());
     rc = convolve(c, ROW_SQR, &jq, 1);
     if (rc == ANNIE_OK) {
-      const CarryJob cq = cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1);
+      const CarryJob cq = cjob(B.wq, B.tq, a, 1, 0, 1);
       rc = launch_carry(c, &cq, 1);
     }
     c.cur = c.stream;
     if (rc) return rc;
     HIPCHK(hipEventRecord(c.ev_join, c.aux));
-    return reduce(c, c.reg(R0), false, nullptr, true);
+    return reduce(c, B, B.y, false, nullptr, true);
   }
-  const NttJob j[2] = {job(c.reg(R0), c.reg(RWR), c.fwdR(), c.invR()), job(c.reg(R0), c.reg(RWQ), c.fwdQ(), c.invQ())};
+  const NttJob j[2] = {job(B.x, B.wr, c.fwdR(), c.invR()), job(B.x, B.wq, c.fwdQ(), c.invQ())};
   if ((rc = convolve(c, ROW_SQR, j, 2))) return rc;
-  const CarryJob cj[2] = {cjob(c.reg(RWR), c.reg(RTR), a, 0, 0, 0), cjob(c.reg(RWQ), c.tq(), a, 1, 0, 1)};
+  const CarryJob cj[2] = {cjob(B.wr, B.tr, a, 0, 0, 0), cjob(B.wq, B.tq, a, 1, 0, 1)};
   if ((rc = launch_carry(c, cj, 2, a_slot))) return rc;
-  return reduce(c, c.reg(R0), false, a_slot);
+  return reduce(c, B, B.y, false, a_slot);
+}
+
+static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
+  return square_mul(c, ctx_bufs(c), a, a_slot);
 }
 
 // mulMMGW (src/transformGPU.h:540-554).
@@ -608,10 +636,12 @@ static void destroy(annie_hip_t* h) {
   if (c.tw) (void)hipFree(c.tw);
   for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
   if (c.graph_buf) (void)hipFree(c.graph_buf);
-  if (c.stage.dev) (void)hipFree(c.stage.dev);
-  if (c.stage.host) (void)hipHostFree(c.stage.host);
-  for (hipEvent_t e : c.stage.ev)
-    if (e) (void)hipEventDestroy(e);
+  for (Stage* st : {&c.stage, &c.run_stage}) {
+    if (st->dev) (void)hipFree(st->dev);
+    if (st->host) (void)hipHostFree(st->host);
+    for (hipEvent_t e : st->ev)
+      if (e) (void)hipEventDestroy(e);
+  }
   for (hipEvent_t e : c.ev) (void)hipEventDestroy(e);
   if (c.stream) (void)hipStreamDestroy(c.stream);
   mpz_clears(c.N, c.factor, nullptr);
@@ -1263,6 +1293,112 @@ int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint
   if ((rc = square_graph(c, count, &exec))) return rc;
   if ((rc = stage_upload(c, slots, a, count, &d, &da, c.graph_buf))) return rc;
   HIPCHK(hipGraphLaunch(exec, c.stream));
+  return ANNIE_OK;
+}
+
+// Kernel of a squaring run for the batch's plan, or null when the plan has
+// none (two-pass lengths, other RNS sizes or digit widths).
+typedef void (*RunKernel)(const RunArgs);
+static RunKernel run_kernel(const Plan& pl, int* threads) {
+  const int qs = pl.cbits() >> 5;
+  RunKernel f = nullptr;
+#define ANNIE_RUN_CASE(M)                                                        \
+  case M:                                                                        \
+    *threads = (((1 << M) / NTT_R) >= 64) ? ((1 << M) / NTT_R) : 64;             \
+    if (pl.K == 3 && qs == 1) f = k_sqr_run<M, 3, 1>;                            \
+    if (pl.K == 4 && qs == 1) f = k_sqr_run<M, 4, 1>;                            \
+    if (pl.K == 5 && qs == 2) f = k_sqr_run<M, 5, 2>;                            \
+    break;
+  switch (pl.LN) {
+    ANNIE_RUN_CASE(9)
+    ANNIE_RUN_CASE(10)
+    ANNIE_RUN_CASE(11)
+    ANNIE_RUN_CASE(12)
+    ANNIE_RUN_CASE(13)
+    default: break;
+  }
+#undef ANNIE_RUN_CASE
+  return f;
+}
+
+int annie_hip_batch_run_supported(const annie_hip_t* h) {
+  if (!h) return 0;
+  int t = 0;
+  return (!h->c.slotN.empty() && run_kernel(h->c.pl, &t) != nullptr) ? 1 : 0;
+}
+
+int annie_hip_batch_square_run(annie_hip_t* h, const uint32_t* slots, size_t count, const uint32_t* mult,
+                               size_t steps) {
+  if (!h || !mult) return fail(ANNIE_ERR_ARG, "null argument");
+  Ctx& c = h->c;
+  if (c.slotN.empty()) return fail(ANNIE_ERR_ARG, "not a batch");
+  int rc;
+  if ((rc = check_slots(c, slots, count))) return rc;
+  if (steps < 1 || steps > ANNIE_HIP_RUN_MAX_STEPS) return fail(ANNIE_ERR_ARG, "steps out of range");
+  int threads = 0;
+  const RunKernel f = run_kernel(c.pl, &threads);
+  if (!f) return fail(ANNIE_ERR_SIZE, "no squaring-run kernel for this plan");
+  // Per slot: multipliers in [1, 2^31) (bit 31: snapshot), zeros only as a
+  // suffix, a snapshot only on the last squaring of the run.
+  for (size_t i = 0; i < count; ++i) {
+    const uint32_t* m = mult + i * steps;
+    size_t len = 0;
+    while (len < steps && m[len] != 0u) ++len;
+    for (size_t t = len; t < steps; ++t)
+      if (m[t] != 0u) return fail(ANNIE_ERR_ARG, "multiplier after the end of a run");
+    for (size_t t = 0; t < len; ++t)
+      if ((m[t] & SLOT_SNAP) && t + 1 != len) return fail(ANNIE_ERR_ARG, "snapshot before the end of a run");
+  }
+  Stage& st = c.run_stage;
+  if (!st.dev) {
+    st.cap = c.nslots_alloc * (ANNIE_HIP_RUN_MAX_STEPS + 1);
+    if (hipMalloc(&st.dev, NSTAGE * st.cap * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "run staging");
+    if (hipHostMalloc(&st.host, NSTAGE * st.cap * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "run staging");
+    for (int e = 0; e < NSTAGE; ++e) HIPCHK(hipEventCreateWithFlags(&st.ev[e], hipEventDisableTiming));
+  }
+  const size_t words = count * (steps + 1);
+  const int e = st.pos;
+  st.pos = (st.pos + 1) % NSTAGE;
+  if (st.pending[e]) {
+    HIPCHK(hipEventSynchronize(st.ev[e]));
+    st.pending[e] = false;
+  }
+  uint32_t* hbuf = st.host + size_t(e) * st.cap;
+  uint32_t* dbuf = st.dev + size_t(e) * st.cap;
+  std::memcpy(hbuf, slots, count * 4);
+  std::memcpy(hbuf + count, mult, count * steps * 4);
+  HIPCHK(hipMemcpyAsync(dbuf, hbuf, words * 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipEventRecord(st.ev[e], c.stream));
+  st.pending[e] = true;
+
+  RunArgs R{};
+  R.nt = c.nbase;
+  R.nt.m = c.pl.LN;
+  R.nt.m1 = 0;
+  R.ca = c.cbase;
+  R.slot_ids = dbuf;
+  R.mult = dbuf + count;
+  R.slot_stride = c.slot_stride;
+  R.steps = int(steps);
+  R.r0 = c.reg(R0);
+  R.wr = c.reg(RWR);
+  R.wq = c.reg(RWQ);
+  R.tr = c.reg(RTR);
+  R.sq = c.reg(RSQ);
+  R.tq = c.tq();
+  R.snap = c.reg(RSNAP);
+  R.nir = c.reg(RNIR);
+  R.nq = c.reg(RNQ);
+  R.fwdR = c.fwdR();
+  R.invR = c.invR();
+  R.fwdQ = c.fwdQ();
+  R.invQ = c.invQ();
+  const uint32_t S = 1u << c.pl.LN;
+  const uint32_t lines = uint32_t(threads) / (S / NTT_R);
+  const size_t shm = std::max<size_t>(size_t(lines) * (S + (S >> 5)), size_t(c.pl.K + 1) * (S / CARRY_L)) * 4;
+  ProfScope ps(c, "k_sqr_run<" + std::to_string(c.pl.LN) + "," + std::to_string(c.pl.K) + ">");
+  hipLaunchKernelGGL(f, dim3(unsigned(count)), dim3(unsigned(threads)), shm, c.stream, R);
+  HIPCHK(hipGetLastError());
   return ANNIE_OK;
 }
 

@@ -79,4 +79,25 @@ struct CarryArgs {
   int cbits;  // bits per coefficient (pck * lbase)
 };
 
+// A run of batched squarings (one workgroup per listed slot): for slot list
+// position i and step t < steps, r0 = m * r0^2 with m = mult[i * steps + t];
+// m = 0 ends the slot's run, bit SLOT_SNAP also writes the new r0 to the
+// snapshot register.  Register pointers address slot 0 (slot s at
+// + s * slot_stride words).
+struct RunArgs {
+  NttArgs nt;            // transform constants (p, q, sc, scq, LN)
+  CarryArgs ca;          // CRT and carry constants (p, q, Garner inverses, cbits)
+  const uint32_t* slot_ids;
+  const uint32_t* mult;
+  size_t slot_stride;
+  int steps;
+  const uint32_t* r0;    // x, and the result of each squaring
+  uint32_t *wr, *wq;     // cyclic / negacyclic convolutions
+  uint32_t *tr, *sq;     // T_R then m' + 2; the (m' + 2) N convolution
+  uint32_t* tq;          // T_Q digits (DIGIT_LIMBS planes)
+  uint32_t* snap;        // snapshot register
+  const uint32_t *nir, *nq;   // transformed N^-1 mod R and N (scaled)
+  const TwDev *fwdR, *invR, *fwdQ, *invQ;
+};
+
 }  // namespace annie

@@ -265,6 +265,83 @@ struct RowGeo {
   static constexpr int WAVES_PER_EU = 4;
 };
 
+// One line of a row pass (the squaring runs below; the launched pass k_row
+// has the same body): levels [m1, LN) of the 2^M elements of job J's
+// plane k at word `base` (slot offset + k n + row 2^M), whose root node is
+// lineRoot, then (by mode) the pointwise product and the inverse levels.  tg
+// is the thread's index among the 2^M / 16 threads of the line, X the line's
+// LDS buffer (2^M + 2^M / 32 words).  Every thread of the workgroup must call
+// it together (it synchronises the workgroup between rounds).
+template <int M, int MODE, bool AH>
+__device__ __forceinline__ void row_line(const NttJob& J, const NttArgs& A, int k, size_t base, uint32_t lineRoot,
+                                         uint32_t tg, uint32_t* X) {
+  constexpr int LR = RowGeo<M>::LR;
+  constexpr int NR = Sched<M, LR>::NR, DL = LR * (NR - 1), RL = Sched<M, LR>::template rr<NR - 1>();
+  const uint32_t p = A.p[k];
+  const size_t n = size_t(1) << A.LN;
+  const TwDev* twf = J.fwd + size_t(k) * n;
+  const TwDev* twi = J.inv + size_t(k) * n;
+  const uint32_t* src = J.src + base;
+  uint32_t* dst = J.dst + base;
+
+  auto gld = [&](uint32_t i) { return src[i]; };
+  auto gst = [&](uint32_t i, uint32_t v) { dst[i] = v; };
+  auto lld = [&](uint32_t i) { return X[lpad(i)]; };
+  auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
+
+  // With AH, each round's roots are loaded one round ahead.
+  uint32_t x[1 << LR];
+  SchedTw<M, LR, 0> t0;
+  if constexpr (AH) round_tw(t0, M, 0, tg, lineRoot, twf);
+  if constexpr (MODE == ROW_FWD) {
+    fwd_rounds<M, LR, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [] {});
+    round_store<LR, RL>(x, M, DL, tg, gst);
+  } else {
+    // With AH the last forward round also issues the loads of the first
+    // inverse round's roots and of the pointwise operand (MULC / MULV);
+    // otherwise the operand is loaded just before the product.
+    SchedTw<M, LR, NR - 1> ti;
+    uint32_t opv[1 << LR];
+    auto load_opv = [&] {
+      if constexpr (MODE != ROW_SQR) {
+        const uint32_t* op = J.opnd + base;
+#pragma unroll
+        for (int s = 0; s < ((1 << LR) >> RL); ++s) {
+          uint32_t node;
+          const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
+#pragma unroll
+          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
+        }
+      }
+    };
+    fwd_rounds<M, LR, 0, AH, false>(x, tg, lineRoot, twf, p, A.q[k], gld, lld, lst, t0, [&] {
+      if constexpr (AH) {
+        round_tw(ti, M, DL, tg, lineRoot, twi);
+        load_opv();
+      }
+    });
+    if constexpr (!AH) load_opv();
+    // Pointwise product on the leaves held in registers.
+    const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
+#pragma unroll
+    for (int i = 0; i < (1 << LR); ++i) {
+      uint32_t& v = x[i];
+      if constexpr (MODE == ROW_SQR) {
+        v = mont_c(mont(v, v, p, q), sc, scq, p);
+      } else if constexpr (MODE == ROW_MULC) {
+        v = mont(v, opv[i], p, q);
+      } else {
+        v = mont_c(mont(v, opv[i], p, q), sc, scq, p);
+      }
+    }
+    inv_rounds<M, LR, NR - 1, AH>(x, tg, lineRoot, twi, p, A.q[k], gst, lld, lst, ti);
+  }
+}
+
+// The launched row pass.  Its body is row_line's, written out: with the
+// inlined call the compiler allocates four more VGPRs and spills in the
+// constant-multiply mode, which this pass (the largest per squaring at the
+// longest lengths) cannot afford.
 template <int M, int MODE>
 __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k_row(const NttArgs A) {
   extern __shared__ uint32_t lds[];
@@ -914,6 +991,181 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   carry_store<K>(J, A, n, b, dig);
 }
 
+// ---------------------------------------------------------------------------
+// Runs of batched squarings: one workgroup owns one slot and runs whole
+// squarings, step after step, with workgroup barriers where the launched
+// path has kernel boundaries.  Each squaring is the launched path's sequence
+// over the slot's registers (L2-resident at these lengths): the cyclic and
+// negacyclic convolutions of x^2 (row_line, one line per prime plane and
+// product), their carries, the m' convolution and carry (canonical, + 2), the
+// (m' + 2) N convolution and the carry that subtracts T_Q.  Lengths 2^9 ..
+// 2^13 (single-pass transforms); the carry keeps CARRY_L coefficients per
+// block and the lazy one-block hand-off of k_carry, so the registers hold
+// the same words as after the launched path.
+
+// Threads per workgroup: one line of 2^M / 16 threads (two lines of 32 at
+// 2^9), and carry blocks per thread.
+template <int M>
+struct RunGeo {
+  static constexpr int TL = (1 << M) / NTT_R;
+  static constexpr int T = TL >= 64 ? TL : 64;
+  static constexpr int LINES = T / TL;
+  static constexpr int NBT = (1 << M) / CARRY_L / T;
+  // Waves per SIMD the register budget is set for: the row stages and the
+  // carry together need up to ~160 VGPRs at 2^9 .. 2^12 (three waves), more
+  // at 2^13 (two waves); a budget of 128 spills heavily.
+  static constexpr int WAVES = (M == 13) ? 2 : 3;
+};
+
+// The carry of one register by one workgroup of T threads: block b = j T + t
+// (j < NBT) to thread t.  Phase 1 of every block: its digits go to the
+// destination as DIGIT_LIMBS raw planes (the destination holds at least
+// three planes and is not the source) and its zero-in carry-out to LDS (cl:
+// L words per block, limb-major).  Phase 2 reads them back and adds the
+// predecessor's carry-out, one block in registers at a time.  For a
+// canonical carry (m' of reduceMMGW) the finished digits go back to the
+// destination, the sign of the number is that of its most significant
+// nonzero digit (a workgroup maximum of (digit index + 1) * 2 + sign bit in
+// `top`), and a third pass adjusts the end blocks as k_carry does and writes
+// the residues.
+template <int K, int QS, int NBT>
+__device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, size_t n, uint32_t T, uint32_t* cl,
+                                         int* top) {
+  constexpr int L = K + 1;
+  const uint32_t s = uint32_t(A.cbits) & 31u;
+  const size_t nb = n / CARRY_L;
+  const uint32_t t = threadIdx.x;
+  CarryJob raw = J;
+  raw.raw_out = 1;
+  raw.dst2 = nullptr;
+  auto load_digits = [&](size_t b, uint32_t (&dig)[CARRY_L][DIGIT_LIMBS]) {
+#pragma unroll
+    for (int l = 0; l < DIGIT_LIMBS; ++l) {
+      const uint4 v = *reinterpret_cast<const uint4*>(J.dst + size_t(l) * n + b * CARRY_L);
+      dig[0][l] = v.x;
+      dig[1][l] = v.y;
+      dig[2][l] = v.z;
+      dig[3][l] = v.w;
+    }
+  };
+  if (t == 0) *top = 0;
+  for (int j = 0; j < NBT; ++j) {
+    const size_t b = size_t(j) * T + t;
+    uint32_t dig[CARRY_L][DIGIT_LIMBS], fo[L];
+    carry_block_local<K, L, QS>(J, A, n, b, s, dig, fo);
+    carry_store<K>(raw, A, n, b, dig);
+#pragma unroll
+    for (int i = 0; i < L; ++i) cl[size_t(i) * nb + b] = fo[i];
+  }
+  __syncthreads();
+  int key = 0;
+  for (int j = 0; j < NBT; ++j) {
+    const size_t b = size_t(j) * T + t;
+    const size_t pb = (b == 0) ? nb - 1 : b - 1;
+    uint32_t dig[CARRY_L][DIGIT_LIMBS], fin[L];
+    load_digits(b, dig);
+#pragma unroll
+    for (int i = 0; i < L; ++i) fin[i] = cl[size_t(i) * nb + pb];
+    if (b == 0 && J.nega) big_neg<L>(fin);
+    carry_block_finish<L, QS>(dig, fin, s);
+    if (J.canon == 1) {
+#pragma unroll
+      for (int jj = CARRY_L - 1; jj >= 0; --jj) {
+        const int sg = digit_sign(dig[jj]);
+        if (sg != 0) {
+          key = max(key, int(b * CARRY_L + jj + 1) * 2 + (sg < 0 ? 1 : 0));
+          break;
+        }
+      }
+      carry_store<K>(raw, A, n, b, dig);
+    } else {
+      carry_store<K>(J, A, n, b, dig);
+    }
+  }
+  if (J.canon != 1) return;
+  atomicMax(top, key);
+  __syncthreads();
+  const int kk = *top;
+  const int sg = (kk == 0) ? 0 : ((kk & 1) ? -1 : 1);
+  for (int j = 0; j < NBT; ++j) {
+    const size_t b = size_t(j) * T + t;
+    uint32_t dig[CARRY_L][DIGIT_LIMBS];
+    load_digits(b, dig);
+    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits);
+    if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
+    carry_store<K>(J, A, n, b, dig);
+  }
+}
+
+// Grid: (slot-list length); block RunGeo<M>::T; dynamic LDS max(LINES lines
+// of 2^M + 2^M / 32 words, (K + 1) n / 4 words).
+template <int M, int K, int QS>
+__global__ void __launch_bounds__(RunGeo<M>::T, RunGeo<M>::WAVES) k_sqr_run(const RunArgs R) {
+  extern __shared__ uint32_t lds[];
+  __shared__ int top;
+  constexpr uint32_t S = 1u << M, Tl = RunGeo<M>::TL;
+  constexpr int LINES = RunGeo<M>::LINES;
+  const size_t n = S;
+  const size_t so = size_t(R.slot_ids[blockIdx.x]) * R.slot_stride;
+  const uint32_t line = threadIdx.x / Tl, tg = threadIdx.x % Tl;
+  uint32_t* X = lds + line * (S + (S >> 5));
+  const uint32_t* mult = R.mult + size_t(blockIdx.x) * size_t(R.steps);
+  // Lines of a stage: `count` planes (job i / K, prime i % K), LINES at a
+  // time; a surplus line repeats the last plane (same words to the same
+  // addresses), so that every thread takes the row pass's barriers.
+  auto stage = [&](auto mode, auto job_of, int count) {
+    constexpr int MODE = decltype(mode)::value;
+    for (int i0 = 0; i0 < count; i0 += LINES) {
+      const int i = min(i0 + int(line), count - 1);
+      const int k = i % K;
+      row_line<M, MODE, false>(job_of(i / K), R.nt, k, size_t(k) * n, 1u, tg, X);
+      __syncthreads();
+    }
+  };
+  auto sq_job = [&](int j) {
+    return j == 0 ? NttJob{R.r0 + so, R.wr + so, R.fwdR, R.invR, nullptr}
+                  : NttJob{R.r0 + so, R.wq + so, R.fwdQ, R.invQ, nullptr};
+  };
+  auto m_job = [&](int) { return NttJob{R.tr + so, R.wr + so, R.fwdR, R.invR, R.nir + so}; };
+  auto mn_job = [&](int) { return NttJob{R.tr + so, R.sq + so, R.fwdQ, R.invQ, R.nq + so}; };
+  for (int step = 0; step < R.steps; ++step) {
+    const uint32_t m = mult[step];
+    if (m == 0u) break;
+    const uint32_t a = m & ~SLOT_SNAP;
+    stage(std::integral_constant<int, ROW_SQR>(), sq_job, 2 * K);
+    CarryJob c{};
+    c.src = R.wr + so;
+    c.dst = R.tr + so;
+    c.a = a;
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    __syncthreads();
+    c.src = R.wq + so;
+    c.dst = R.tq + so;
+    c.nega = 1;
+    c.raw_out = 1;
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    __syncthreads();
+    stage(std::integral_constant<int, ROW_MULC>(), m_job, K);
+    c = CarryJob{};
+    c.src = R.wr + so;
+    c.dst = R.tr + so;
+    c.a = 1u;
+    c.canon = 1;
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    __syncthreads();
+    stage(std::integral_constant<int, ROW_MULC>(), mn_job, K);
+    c = CarryJob{};
+    c.src = R.sq + so;
+    c.dst = const_cast<uint32_t*>(R.r0) + so;
+    c.sub = R.tq + so;
+    c.a = 1u;
+    c.nega = 1;
+    c.dst2 = (m & SLOT_SNAP) ? R.snap + so : nullptr;
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    __syncthreads();
+  }
+}
+
 // Digits (DIGIT_LIMBS planes, two's complement, |d| < 2^95) -> K residue
 // planes, CARRY_L digits per thread: the RNS encode of set_mpz
 // (src/transformGPU.h:323-360) on the device.
@@ -988,6 +1240,17 @@ ANNIE_CARRY(2)
 ANNIE_CARRY(3)
 ANNIE_CARRY(4)
 ANNIE_CARRY(5)
+// Runs: the classes of lengths 2^9 .. 2^13 (K = 3, 4 with digits of 33..63
+// bits; K = 5 with digits of 64..95 bits).
+#define ANNIE_RUN(M)                                          \
+  template __global__ void k_sqr_run<M, 3, 1>(const RunArgs); \
+  template __global__ void k_sqr_run<M, 4, 1>(const RunArgs); \
+  template __global__ void k_sqr_run<M, 5, 2>(const RunArgs);
+ANNIE_RUN(9)
+ANNIE_RUN(10)
+ANNIE_RUN(11)
+ANNIE_RUN(12)
+ANNIE_RUN(13)
 template __global__ void k_digits_to_res<2>(const uint32_t*, uint32_t*, const CarryArgs);
 template __global__ void k_digits_to_res<3>(const uint32_t*, uint32_t*, const CarryArgs);
 template __global__ void k_digits_to_res<4>(const uint32_t*, uint32_t*, const CarryArgs);

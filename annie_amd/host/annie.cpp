@@ -236,8 +236,25 @@ static int check(const mpz_t n, const annie_check_opts& o, annie_check_result& r
 
   auto elapsed = [&]() { return restored + std::chrono::duration<double>(clock::now() - t_start).count(); };
   auto last_ckpt = clock::now();
+  // Progress line every 10 s of display time (src/annie.h:103-120, 328-331):
+  // fraction done, remaining / expected time, ms per loop iteration.
+  const int i_top = int(mpz_sizeinbase(e, 2)) - 1;
+  auto last_disp = clock::now();
+  int disp_i = i_start;
+  auto progress = [&](int i) {
+    const double dt = std::chrono::duration<double>(clock::now() - last_disp).count();
+    if (dt < 10.0 || disp_i == i) return;
+    const double frac = double(i_top - i) / double(std::max(i_top, 1));
+    const double per = dt / double(disp_i - i);
+    std::printf("%.3g%% done, %.0f s / %.0f s remaining, %.4g ms/bit.\n", frac * 100.0, per * i,
+                frac > 0 ? elapsed() / frac : 0.0, per * 1e3);
+    std::fflush(stdout);
+    disp_i = i;
+    last_disp = clock::now();
+  };
   long iters = 0;
   for (int i = i_start; i >= 0; --i) {  // hot loop, src/annie.h:319-341
+    if (o.verbose && (i & 1023) == 0) progress(i);
     if (g_quit.load(std::memory_order_relaxed) || (o.quit_after >= 0 && iters == o.quit_after)) {
       if (!cpath.empty() && T->has_checkpoint()) write_checkpoint(cpath, i, elapsed(), T->save_checkpoint());
       r.status = 2;

@@ -59,6 +59,10 @@ class BatchEngine {
   virtual void square_mul(const std::vector<uint32_t>& s, const std::vector<uint32_t>& a) = 0;
   virtual void mul(int dst, int src, const std::vector<uint32_t>& s) = 0;
   virtual void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) = 0;
+  // Squaring runs: slot s[i] takes the multipliers mult[i * steps ...] up to
+  // its first zero (annie_hip_batch_square_run).
+  virtual bool can_run() const { return false; }
+  virtual void square_run(const std::vector<uint32_t>& s, const std::vector<uint32_t>& mult, size_t steps) = 0;
 };
 
 // The gfx950 engine's batch C ABI.
@@ -104,6 +108,10 @@ class HipBatch : public BatchEngine {
   void mul(int dst, int src, const std::vector<uint32_t>& s) override {
     check(annie_hip_batch_mul(h_, size_t(dst), size_t(src), s.data(), s.size()), "mul");
   }
+  bool can_run() const override { return annie_hip_batch_run_supported(h_) != 0; }
+  void square_run(const std::vector<uint32_t>& s, const std::vector<uint32_t>& mult, size_t steps) override {
+    check(annie_hip_batch_square_run(h_, s.data(), s.size(), mult.data(), steps), "square_run");
+  }
   void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) override {
     limbs_.assign(stride_ * s.size(), 0);
     nout_.assign(s.size(), 0);
@@ -148,6 +156,11 @@ class GmpBatch : public BatchEngine {
   }
   void get(int reg, const std::vector<uint32_t>& s, std::vector<Big*>& out) override {
     for (size_t j = 0; j < s.size(); ++j) t_[s[j]]->get(out[j]->v, size_t(reg));
+  }
+  bool can_run() const override { return true; }
+  void square_run(const std::vector<uint32_t>& s, const std::vector<uint32_t>& mult, size_t steps) override {
+    for (size_t j = 0; j < s.size(); ++j)
+      for (size_t t = 0; t < steps && mult[j * steps + t] != 0u; ++t) square_mul({s[j]}, {mult[j * steps + t]});
   }
 };
 
@@ -435,7 +448,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
   std::vector<std::unique_ptr<Big>> store;
   // Per-tick operation lists, reused across ticks and classes (cleared, not
   // reallocated, every tick).
-  std::vector<uint32_t> set1, sq, sqa;
+  std::vector<uint32_t> set1, sq, sqa, run_ids, run_mult;
   std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
   std::map<int, std::vector<uint32_t>> gt;
   std::vector<uint8_t> handled;
@@ -519,12 +532,43 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       for (auto& g : ml) g.second.clear();
       for (auto& g : gt) g.second.clear();
       // Deferred Gerbicz-Li products of all tests are taken together every
-      // FLUSH ticks (a test forces its own earlier if it must).
-      const bool flush = (++c.ticks % flush_ticks) == 0;
+      // FLUSH ticks (a test forces its own earlier if it must).  In run
+      // mode a tick is many squarings long: pending products are taken at
+      // every tick, and each test whose next operation is a squaring takes
+      // its following squarings (up to run_steps, ending at one that takes
+      // a Gerbicz-Li snapshot or before any other operation) in one launch.
+      const bool run_mode = o.run_steps > 0 && c.active >= std::max<size_t>(size_t(o.run_min_active), 1) &&
+                            c.eng->can_run();
+      const bool flush = run_mode || (++c.ticks % flush_ticks) == 0;
+      const size_t steps = size_t(std::max(o.run_steps, 1));
+      run_ids.clear();
+      run_mult.clear();
+      handled.assign(c.slots, 0);
       for (size_t s = 0; s < c.slots; ++s) {
         if (!c.slot[s]) continue;
-        const Op op = c.slot[s]->m->op(flush);
+        TestMachine& m = *c.slot[s]->m;
+        const Op op = m.op(flush);
         const uint32_t id = uint32_t(s);
+        if (run_mode && op.kind == OP_SQR) {
+          const size_t row = run_mult.size();
+          run_ids.push_back(id);
+          run_mult.resize(row + steps, 0u);
+          run_mult[row] = op.a;
+          m.advance(none.v);
+          size_t k = 1;
+          if (!(op.a & SNAP)) {
+            while (k < steps) {
+              const Op nx = m.op(false);
+              if (nx.kind != OP_SQR) break;
+              run_mult[row + k++] = nx.a;
+              m.advance(none.v);
+              if (nx.a & SNAP) break;
+            }
+          }
+          squarings += double(k);
+          handled[s] = 1;
+          continue;
+        }
         switch (op.kind) {
           case OP_SET1: set1.push_back(id); break;
           case OP_COPY: cp[{op.dst, op.src}].push_back(id); break;
@@ -544,9 +588,9 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
         c.eng->square_mul(sq, sqa);
         squarings += double(sq.size());
       }
+      if (!run_ids.empty()) c.eng->square_run(run_ids, run_mult, steps);
       for (auto& g : ml)
         if (!g.second.empty()) c.eng->mul(g.first.first, g.first.second, g.second);
-      handled.assign(c.slots, 0);
       for (auto& g : gt) {
         if (g.second.empty()) continue;
         store.clear();

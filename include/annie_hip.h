@@ -142,6 +142,18 @@ int annie_hip_batch_copy(annie_hip_t* b, size_t dst, size_t src, const uint32_t*
 int annie_hip_batch_square_mul(annie_hip_t* b, const uint32_t* slots, const uint32_t* a, size_t count);
 /* r_dst *= r_src; src may be 3, the snapshot register. */
 int annie_hip_batch_mul(annie_hip_t* b, size_t dst, size_t src, const uint32_t* slots, size_t count);
+/* A run of squarings in one launch (src/annie.h:319-341 over many loop
+ * iterations): for each listed slot i, r0 = m * r0^2 for m = mult[i * steps + t],
+ * t = 0, 1, ..., up to the slot's first m = 0 (zeros only as a suffix); bit 31
+ * of m as in annie_hip_batch_square_mul, allowed on the last squaring of the
+ * slot's run only.  One workgroup per slot runs the whole squaring sequence;
+ * the registers end equal, word for word, to those of the same squarings
+ * issued one by one.  steps <= ANNIE_HIP_RUN_MAX_STEPS.  ANNIE_ERR_SIZE when
+ * the batch's plan has no run kernel (annie_hip_batch_run_supported == 0:
+ * transform lengths outside 2^9 .. 2^13, other RNS sizes or digit widths). */
+#define ANNIE_HIP_RUN_MAX_STEPS 256
+int annie_hip_batch_square_run(annie_hip_t* b, const uint32_t* slots, size_t count, const uint32_t* mult, size_t steps);
+int annie_hip_batch_run_supported(const annie_hip_t* b);
 /* Canonical values of register `reg` of each listed slot: slot i's limbs at
  * limbs + i * stride (stride limbs each), nout[i] limbs used. */
 int annie_hip_batch_get(annie_hip_t* b, size_t reg, const uint32_t* slots, size_t count, uint64_t* limbs, size_t stride,

@@ -77,6 +77,10 @@ typedef struct {
   int verbose;                /* print each result line on stderr as it is decided */
   int graphs;                 /* 1: batched squarings replay captured HIP graphs */
   int flush_ticks;            /* deferred Gerbicz-Li products taken every that many ticks, 0 = 16 */
+  int run_steps;              /* > 0: a class with at least run_min_active running tests takes up to
+                                 that many squarings per test in one launch (squaring runs, lengths
+                                 2^9 .. 2^13); 0: one squaring per tick */
+  int run_min_active;         /* running tests a class needs for squaring runs, 0 = 1 */
 } annie_batch_opts;
 
 typedef struct {

@@ -49,6 +49,20 @@ def test_batch_matches_oracle(max_slots):
     assert st["squarings"] >= sum(r["iterations"] for r in res)
 
 
+@pytest.mark.parametrize("run_steps,min_active", [(1, 1), (8, 1), (64, 3)])
+def test_squaring_runs_match_oracle(run_steps, min_active):
+    """Run mode (up to run_steps squarings per test per tick, ending at a
+    Gerbicz-Li snapshot) issues the same square_mul sequence per test."""
+    tests = _mix(12, 16)
+    base, st0 = annie_amd.batch_run(tests, engine="gmp", max_slots=5)
+    res, st = annie_amd.batch_run(tests, engine="gmp", max_slots=5, run_steps=run_steps, run_min_active=min_active)
+    assert st["rc"] == 0
+    _assert_matches(tests, res)
+    assert st["squarings"] == st0["squarings"]
+    for a, b in zip(base, res):
+        assert (a["iterations"], a["res64"], a["is_prp"]) == (b["iterations"], b["res64"], b["is_prp"])
+
+
 def test_known_proth_primes_in_a_batch():
     # 3*2^n+1 is prime for n = 2208, 2816, 3168 (OEIS A002253); 3*2^2209+1 is not.
     tests = [{"N": proth(3, n), "is_euler": True} for n in (2208, 2816, 3168, 2209)]
