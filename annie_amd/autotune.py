@@ -6,7 +6,10 @@ Knobs (all leave results unchanged; only the schedule differs):
                launch by launch;
   flush_ticks  period (in ticks) at which the deferred Gerbicz-Li products of
                all tests of the class are taken together;
-  max_slots    tests resident at once (all of the class, or a fraction).
+  max_slots    tests resident at once (all of the class, or a fraction);
+  run_steps    squarings per test per launch in squaring runs (one workgroup
+               per test runs whole squarings; 0 = one squaring per launch),
+               for classes of transform lengths 2^9 .. 2^13.
 
 A trial runs the class's own tests for `seconds` of square_mul work (batch
 set-up excluded) and scores square_mul operations per second.  The cache key
@@ -28,8 +31,11 @@ def _device_name():
 
 def candidates(count):
     slots = sorted({count, max(1, count // 2)})
-    return [{"graphs": g, "flush_ticks": f, "max_slots": s}
-            for g, f, s in itertools.product((False, True), (8, 16, 64), slots)]
+    launched = [{"graphs": g, "flush_ticks": f, "max_slots": s, "run_steps": 0}
+                for g, f, s in itertools.product((False, True), (8, 16, 64), slots)]
+    runs = [{"graphs": False, "flush_ticks": 16, "max_slots": s, "run_steps": r, "run_min_active": 1}
+            for r, s in itertools.product((8, 32, 128), slots)]
+    return launched + runs
 
 
 def tune_class(tests, seconds=0.5, device=0, configs=None, engine="gpu"):

@@ -79,6 +79,58 @@ ANNIE_HD uint32_t submul_c(uint32_t a, uint32_t b, uint32_t w, uint32_t wq, uint
   return mont_c(submod(a, b, p), w, wq, p);
 }
 
+// Constants fixed by the five primes, evaluated at compile time so that the
+// CRT / carry kernels (whose prime indices are compile-time after unrolling)
+// take them as instruction literals instead of kernel-argument registers:
+// q = p^-1 mod 2^32; c32 = 2^64 mod p and c64 = 2^96 mod p (mont by them
+// multiplies by 2^32 and 2^64); c32n = p - c32; the products of each with q;
+// ginv[i][j] = p_i^-1 mod p_j in Montgomery form (i != j) and its product
+// with p_j^-1 mod 2^32.  The same values as the host's make_consts.
+struct PrimeLiterals {
+  uint32_t q[KMAX], c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX], c32n[KMAX], c32nq[KMAX];
+  uint32_t ginv[KMAX][KMAX], ginvq[KMAX][KMAX];
+};
+
+constexpr uint32_t cx_powmod(uint32_t b, uint64_t e, uint32_t p) {
+  uint64_t r = 1 % p, x = b % p;
+  while (e) {
+    if (e & 1) r = (r * x) % p;
+    x = (x * x) % p;
+    e >>= 1;
+  }
+  return uint32_t(r);
+}
+
+constexpr uint32_t cx_inv32(uint32_t p) {
+  uint32_t q = p;
+  for (int i = 0; i < 5; ++i) q *= 2u - p * q;
+  return q;
+}
+
+constexpr PrimeLiterals make_prime_literals() {
+  PrimeLiterals c{};
+  for (int k = 0; k < KMAX; ++k) {
+    const uint32_t p = PRIMES[k], q = cx_inv32(p);
+    c.q[k] = q;
+    c.c32[k] = cx_powmod(2, 64, p);
+    c.c32q[k] = c.c32[k] * q;
+    c.c64[k] = cx_powmod(2, 96, p);
+    c.c64q[k] = c.c64[k] * q;
+    c.c32n[k] = p - c.c32[k];
+    c.c32nq[k] = c.c32n[k] * q;
+    for (int j = 0; j < KMAX; ++j) {
+      if (j == k) continue;
+      const uint32_t pj = PRIMES[j];
+      const uint32_t inv = cx_powmod(p % pj, uint64_t(pj) - 2, pj);
+      c.ginv[k][j] = uint32_t((uint64_t(inv) << 32) % pj);
+      c.ginvq[k][j] = c.ginv[k][j] * cx_inv32(pj);
+    }
+  }
+  return c;
+}
+
+constexpr PrimeLiterals PLIT = make_prime_literals();
+
 // Host-side helpers (plain integer arithmetic; used for tables and constants).
 static inline uint32_t powmod_plain(uint32_t b, uint64_t e, uint32_t p) {
   uint64_t r = 1 % p, x = b % p;

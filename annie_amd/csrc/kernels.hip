@@ -36,10 +36,21 @@ __device__ __forceinline__ size_t slot_offset(const uint32_t* ids, size_t stride
   return ids ? size_t(ids[pos]) * stride : size_t(0);
 }
 
+// Forward (Cooley-Tukey) butterfly on lazily reduced residues: a and b are
+// any words congruent to the values mod p (below 2^32; the primes are above
+// 2^31, so a word holds at most two representatives), and so are the
+// outputs.  t = r b 2^-32 mod p is in [0, p) for any b < 2^32.  a - t: one
+// correction (+ p on borrow) leaves [0, 2^32).  a + t: on a carry out of 32
+// bits the wrapped sum s < p gets 2^32 mod p = 2^32 - p added, which stays
+// below 2^32 -- three instructions instead of the four of a full addmod.
+// The inverse butterflies, whose outputs feed the CRT, take and give fully
+// reduced residues; every product (mont, mont_c) accepts lazy operands.
 __device__ __forceinline__ void fwd_bfly(uint32_t& a, uint32_t& b, const Tw2 w, const uint32_t p) {
   const uint32_t t = mont_c(b, w.w, w.wq, p);
+  uint32_t s;
+  const bool carry = __builtin_add_overflow(a, t, &s);
   b = submod(a, t, p);
-  a = addmod(a, t, p);
+  a = s + (carry ? (0u - p) : 0u);
 }
 
 __device__ __forceinline__ void inv_bfly(uint32_t& a, uint32_t& b, const Tw2 w, const uint32_t p) {
@@ -680,15 +691,15 @@ __device__ __forceinline__ void garner(const uint32_t (&r)[K], const CarryArgs& 
   for (int j = K - 1; j >= 0; --j) {
     uint32_t v = r[j];
 #pragma unroll
-    for (int i = K - 1; i > j; --i) v = submul_c(v, u[i], A.ginv[i][j], A.ginvq[i][j], A.p[j]);
+    for (int i = K - 1; i > j; --i) v = submul_c(v, u[i], PLIT.ginv[i][j], PLIT.ginvq[i][j], PRIMES[j]);
     u[j] = v;
   }
-  const int32_t i0 = (u[0] > (A.p[0] >> 1)) ? int32_t(u[0] - A.p[0]) : int32_t(u[0]);
+  const int32_t i0 = (u[0] > (PRIMES[0] >> 1)) ? int32_t(u[0] - PRIMES[0]) : int32_t(u[0]);
   acc[0] = uint32_t(i0);
   uint32_t sg = uint32_t(i0 >> 31);
 #pragma unroll
   for (int j = 1; j < K; ++j) {
-    const uint32_t pj = A.p[j];
+    const uint32_t pj = PRIMES[j];
     uint64_t c = u[j];
 #pragma unroll
     for (int i = 0; i < j; ++i) {
@@ -707,16 +718,16 @@ __device__ __forceinline__ void garner(const uint32_t (&r)[K], const CarryArgs& 
 // and sign neg: u = m 2^-32 mod p by one Montgomery reduction (m - (m0 p^-1
 // mod 2^32) p is a multiple of 2^32), then u * (+-2^64) * 2^-32.
 __device__ __forceinline__ uint32_t digit_residue64(uint32_t m0, uint32_t m1, bool neg, int k, const CarryArgs& A) {
-  const uint32_t p = A.p[k];
-  const uint32_t u = submod(m1, mulhi32(m0 * A.q[k], p), p);
-  const uint32_t w = neg ? A.c32n[k] : A.c32[k];
-  const uint32_t wq = neg ? A.c32nq[k] : A.c32q[k];
+  const uint32_t p = PRIMES[k];
+  const uint32_t u = submod(m1, mulhi32(m0 * PLIT.q[k], p), p);
+  const uint32_t w = neg ? PLIT.c32n[k] : PLIT.c32[k];
+  const uint32_t wq = neg ? PLIT.c32nq[k] : PLIT.c32q[k];
   return mont_c(u, w, wq, p);
 }
 
 // Residue of a signed digit (|d| < 2^95) modulo p_k.
 __device__ __forceinline__ uint32_t digit_residue_wide(const uint32_t (&d)[DIGIT_LIMBS], int k, const CarryArgs& A) {
-  const uint32_t p = A.p[k];
+  const uint32_t p = PRIMES[k];
   const bool neg = int32_t(d[2]) < 0;
   uint32_t a0 = d[0], a1 = d[1], a2 = d[2];
   if (neg) {  // |d| = ~d + 1
@@ -730,8 +741,8 @@ __device__ __forceinline__ uint32_t digit_residue_wide(const uint32_t (&d)[DIGIT
     a2 += uint32_t(t1 >> 32);
   }
   uint32_t r = (a0 >= p) ? a0 - p : a0;
-  r = addmod(r, mont_c(a1, A.c32[k], A.c32q[k], p), p);
-  r = addmod(r, mont_c(a2, A.c64[k], A.c64q[k], p), p);
+  r = addmod(r, mont_c(a1, PLIT.c32[k], PLIT.c32q[k], p), p);
+  r = addmod(r, mont_c(a2, PLIT.c64[k], PLIT.c64q[k], p), p);
   return neg ? submod(0u, r, p) : r;
 }
 
