@@ -27,6 +27,9 @@ ANNIE_ERR = -1
 ANNIE_ERR_SIZE = -2
 ANNIE_ERR_ARG = -3
 ANNIE_ERR_NOMEM = -4
+# Largest K * n words per register of annie_hip_square_mul_noalloc
+# (ANNIE_HIP_NOALLOC_MAX_KN in include/annie_hip.h).
+NOALLOC_MAX_KN = 1 << 19
 
 _lib = None
 
@@ -66,6 +69,7 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_copy.argtypes = [h, sz, sz]
     lib.annie_hip_square_mul.argtypes = [h, ctypes.c_int32]
     lib.annie_hip_square_mul_bits.argtypes = [h, ctypes.POINTER(ctypes.c_uint8), sz, ctypes.c_int32]
+    lib.annie_hip_square_mul_noalloc.argtypes = [h, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32]
     lib.annie_hip_mul.argtypes = [h, sz, sz]
     lib.annie_hip_get.argtypes = [h, sz, u64p, sz, ctypes.POINTER(sz)]
     lib.annie_hip_checkpoint_size.argtypes = [h]
@@ -194,6 +198,11 @@ class Transform:
 
     def square_mul(self, a):
         self._check(self._lib.annie_hip_square_mul(self._h, int(a)))
+
+    def square_mul_noalloc(self, x_ptr, y_ptr, a):
+        """y = a * x^2 on caller-owned device buffers (K planes of n u32 words,
+        plane-major) with the library's static work array; allocates nothing."""
+        self._check(self._lib.annie_hip_square_mul_noalloc(self._h, int(x_ptr), int(y_ptr), int(a)))
 
     def square_mul_bits(self, bits, a):
         """r0 = (bit ? a : 1) * r0^2 for each bit of `bits` (bytes or list of 0/1)."""

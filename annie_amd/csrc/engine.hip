@@ -13,6 +13,7 @@
 #include <cstring>
 #include <algorithm>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -864,6 +865,35 @@ static bool ready(const annie_hip_t* h) {
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Squaring without allocation: the work registers of one squaring (four
+// registers of K n words and the T_Q digits) in a file-scope device array,
+// one copy per GPU, shared by every call on that GPU.  In host code
+// `scratch` only names the symbol; its device address on each GPU comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+constexpr size_t NOALLOC_KN_MAX = ANNIE_HIP_NOALLOC_MAX_KN;
+constexpr size_t SCRATCH_WORDS = 4 * NOALLOC_KN_MAX + size_t(DIGIT_LIMBS) * (NOALLOC_KN_MAX / 2);
+static __device__ __align__(256) uint32_t scratch[SCRATCH_WORDS];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device `dev`.
+static int scratch_on(int dev, uint32_t** out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(ANNIE_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(ANNIE_ERR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return ANNIE_OK;
+}
+
 }  // namespace annie
 
 using namespace annie;
@@ -948,6 +978,31 @@ int annie_hip_square_mul(annie_hip_t* h, int32_t a) {
   if (!ready(h)) return ANNIE_ERR_ARG;
   if (a < 1) return fail(ANNIE_ERR_ARG, "multiplier must be positive");
   return annie::square_mul(h->c, uint32_t(a));
+}
+
+int annie_hip_square_mul_noalloc(annie_hip_t* h, const uint32_t* x, uint32_t* y, int32_t a) {
+  if (!ready(h)) return ANNIE_ERR_ARG;
+  Ctx& c = h->c;
+  if (a < 1) return fail(ANNIE_ERR_ARG, "multiplier must be positive");
+  if (!x || !y) return fail(ANNIE_ERR_ARG, "null register buffer");
+  if ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15u)
+    return fail(ANNIE_ERR_ARG, "register buffers must be 16-byte aligned");
+  if (c.nslots_alloc != 1) return fail(ANNIE_ERR_ARG, "not a single-modulus context");
+  const size_t kn = size_t(c.pl.K) * c.n;
+  if (kn > NOALLOC_KN_MAX || 2 * c.n > NOALLOC_KN_MAX)
+    return fail(ANNIE_ERR_SIZE, "transform of " + std::to_string(kn) + " words per register exceeds the no-allocation "
+                                "limit of " + std::to_string(NOALLOC_KN_MAX));
+  uint32_t* w = nullptr;
+  HIPCHK(hipSetDevice(c.device));
+  int rc;
+  if ((rc = scratch_on(c.device, &w))) return rc;
+  const SqBufs B{x, y, w, w + kn, w + 2 * kn, w + 3 * kn, w + 4 * kn};
+  // One stream: the two-stream squaring is only used above this size limit.
+  hipStream_t aux = c.aux;
+  c.aux = nullptr;
+  rc = annie::square_mul(c, B, uint32_t(a));
+  c.aux = aux;
+  return rc;
 }
 
 int annie_hip_square_mul_bits(annie_hip_t* h, const uint8_t* bits, size_t count, int32_t a) {

@@ -67,6 +67,22 @@ int annie_hip_copy(annie_hip_t* ctx, size_t dst, size_t src);
 /* transform::square_mul -- r0 = a * r0^2 (src/transform.h:28; src/transformGPU.h:593-596). */
 int annie_hip_square_mul(annie_hip_t* ctx, int32_t a);
 
+/* No counterpart in the reference: a squaring that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  y = a * x^2 in MMGW form, the squaring of
+ * annie_hip_square_mul on caller-owned device buffers: x and y hold K planes of
+ * n u32 residues (plane-major, the layout of annie_hip_read_register), 16-byte
+ * aligned; y may equal x.  Its work registers live in a static device array
+ * of the library (one per GPU, 4 K n + 3 n words at the largest size, held
+ * until the process ends) instead of the context's own, so the context's
+ * registers are untouched.  Limit: K * n <= ANNIE_HIP_NOALLOC_MAX_KN words per
+ * register (ANNIE_ERR_SIZE above it).  Enqueued on the context's stream and
+ * returns without waiting.  Every call on one GPU shares that array: calls
+ * whose work overlaps on one GPU (other threads, other contexts' streams)
+ * race; calls on one stream are ordered. */
+#define ANNIE_HIP_NOALLOC_MAX_KN ((size_t)1 << 19)
+int annie_hip_square_mul_noalloc(annie_hip_t* ctx, const uint32_t* x, uint32_t* y, int32_t a);
+
 /* The reference's hot loop body for `count` exponent bits (src/annie.h:319-341 without
  * the Gerbicz-Li products): for i < count, r0 = (bits[i] ? a : 1) * r0^2. */
 int annie_hip_square_mul_bits(annie_hip_t* ctx, const uint8_t* bits, size_t count, int32_t a);
