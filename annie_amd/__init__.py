@@ -367,7 +367,7 @@ class BatchOpts(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int), ("device", ctypes.c_int), ("max_slots", ctypes.c_size_t),
                 ("hbm_budget_mb", ctypes.c_double), ("realtime", ctypes.c_int), ("time_limit", ctypes.c_double),
                 ("verbose", ctypes.c_int), ("graphs", ctypes.c_int), ("flush_ticks", ctypes.c_int),
-                ("run_steps", ctypes.c_int), ("run_min_active", ctypes.c_int)]
+                ("run_steps", ctypes.c_int), ("run_min_active", ctypes.c_int), ("host_threads", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -383,7 +383,7 @@ class BatchStats(ctypes.Structure):
 
 
 def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
-              verbose=False, graphs=False, flush_ticks=0, run_steps=0, run_min_active=0):
+              verbose=False, graphs=False, flush_ticks=0, run_steps=0, run_min_active=0, host_threads=0):
     """Run many tests together, one engine batch per length class, with
     continuous admission (annie_batch_run in include/annie_host.h).
 
@@ -392,7 +392,8 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
     "gmp" runs the same driver on exact GMP transforms on the CPU.
     run_steps > 0: classes with at least run_min_active running tests take up
     to run_steps squarings per test in one launch (squaring runs, lengths
-    2^9 .. 2^13); results are the same either way.
+    2^9 .. 2^13); results are the same either way.  host_threads: host
+    workers driving the classes (0 = one per class, at most 16).
     Returns (results, stats): one dict per test (status, is_prp, res64, gl_ok,
     a, iterations, arrival, admitted, finished, message) and a stats dict.
     """
@@ -410,7 +411,7 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
         arr[i] = BatchTest(limbs, n, 1 if t.get("is_euler") else 0, int(t.get("a", 0)), float(t.get("arrival", 0.0)))
     opts = BatchOpts(0 if engine == "gpu" else 1, device, max_slots, hbm_budget_mb, 1 if realtime else 0,
                      time_limit, 1 if verbose else 0, 1 if graphs else 0, int(flush_ticks), int(run_steps),
-                     int(run_min_active))
+                     int(run_min_active), int(host_threads))
     res = (BatchResult * max(len(tests), 1))()
     st = BatchStats()
     rc = lib.annie_batch_run(arr, len(tests), ctypes.byref(opts), res, ctypes.byref(st))

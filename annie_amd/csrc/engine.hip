@@ -112,6 +112,10 @@ struct Ctx {
   Stage stage;
   Stage run_stage;  // slot lists and multipliers of squaring runs (allocated on first use)
   std::vector<std::unique_ptr<Mpz>> slotN;  // modulus of each slot (batches)
+  // Bits per coefficient of each slot (batches: moduli of one (K, LN) whose
+  // plans differ in lbase share a batch), and their device copy.
+  std::vector<int> slot_cb;
+  uint32_t* cbits_dev = nullptr;
   // Batched squaring replayed from captured HIP graphs: the slot list and
   // multipliers live at fixed device addresses (graph_buf: ids, then
   // multipliers), one executable graph per list length (grid sizes).
@@ -152,6 +156,9 @@ struct annie_hip {
 };
 
 namespace annie {
+
+// Bits per coefficient of `slot` (the plan's, without per-slot values).
+static int slot_cbits(const Ctx& c, size_t slot) { return c.slot_cb.empty() ? c.pl.cbits() : c.slot_cb[slot]; }
 
 static NttJob job(const uint32_t* src, uint32_t* dst, const TwDev* f, const TwDev* i, const uint32_t* op = nullptr) {
   NttJob j;
@@ -471,9 +478,8 @@ static int copy_reg(Ctx& c, int dst, int src) {
 }
 
 // Signed digits (DIGIT_LIMBS planes, two's complement) -> integer sum d_i B^i.
-static void digits_to_mpz(const Ctx& c, const std::vector<uint32_t>& d, mpz_t z) {
+static void digits_to_mpz(const Ctx& c, const std::vector<uint32_t>& d, int cb, mpz_t z) {
   const size_t n = c.n;
-  const int cb = c.pl.cbits();
   const size_t nbits = n * size_t(cb);
   const size_t nl = nbits / 64 + 3;
   std::vector<uint64_t> limbs(nl, 0);
@@ -513,9 +519,8 @@ static uint32_t* slot_reg(Ctx& c, int r, size_t slot) { return c.reg(r) + slot *
 
 // Coefficients of x in base B = 2^cbits as DIGIT_LIMBS planes of 32-bit limbs
 // (bits [i*cbits, (i+1)*cbits) of x, non-negative).
-static void mpz_to_digit_planes(const Ctx& c, const mpz_t x, std::vector<uint32_t>& out) {
+static void mpz_to_digit_planes(const Ctx& c, const mpz_t x, int cb, std::vector<uint32_t>& out) {
   const size_t n = c.n;
-  const int cb = c.pl.cbits();
   out.assign(size_t(DIGIT_LIMBS) * n, 0);
   const size_t xs = mpz_size(x);
   const mp_limb_t* xp = mpz_limbs_read(x);
@@ -540,7 +545,7 @@ static void mpz_to_digit_planes(const Ctx& c, const mpz_t x, std::vector<uint32_
 // device encodes them into K residue planes.
 static int upload_mpz(Ctx& c, int r, const mpz_t x, size_t slot = 0) {
   std::vector<uint32_t> h;
-  mpz_to_digit_planes(c, x, h);
+  mpz_to_digit_planes(c, x, slot_cbits(c, slot), h);
   uint32_t* digits = c.rawtmp() + slot * c.slot_stride;
   HIPCHK(hipMemcpyAsync(digits, h.data(), h.size() * 4, hipMemcpyHostToDevice, c.stream));
   CarryArgs A = c.cbase;
@@ -585,7 +590,7 @@ static int get_values(Ctx& c, int src, const size_t* slots, const mpz_t* const* 
   HIPCHK(hipStreamSynchronize(c.stream));
   for (size_t i = 0; i < count; ++i) {
     const std::vector<uint32_t> di(d.begin() + i * w, d.begin() + (i + 1) * w);
-    digits_to_mpz(c, di, *zs[i]);
+    digits_to_mpz(c, di, slot_cbits(c, slots[i]), *zs[i]);
     mpz_mod(*zs[i], *zs[i], *mods[i]);
   }
   return ANNIE_OK;
@@ -637,6 +642,7 @@ static void destroy(annie_hip_t* h) {
   if (c.tw) (void)hipFree(c.tw);
   for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
   if (c.graph_buf) (void)hipFree(c.graph_buf);
+  if (c.cbits_dev) (void)hipFree(c.cbits_dev);
   for (Stage* st : {&c.stage, &c.run_stage}) {
     if (st->dev) (void)hipFree(st->dev);
     if (st->host) (void)hipHostFree(st->host);
@@ -649,12 +655,14 @@ static void destroy(annie_hip_t* h) {
   delete h;
 }
 
-// R = B^n - 1 of a plan.
-static void plan_R(const Plan& pl, mpz_t R) {
+// R = B^n - 1 for B = 2^cbits, n = 2^LN.
+static void plan_R(int cbits, int LN, mpz_t R) {
   mpz_set_ui(R, 1);
-  mpz_mul_2exp(R, R, mp_bitcnt_t(pl.cbits()) << pl.LN);
+  mpz_mul_2exp(R, R, mp_bitcnt_t(cbits) << LN);
   mpz_sub_ui(R, R, 1);
 }
+
+static void plan_R(const Plan& pl, mpz_t R) { plan_R(pl.cbits(), pl.LN, R); }
 
 // Device state shared by the slots of a context: stream, register file of
 // `slots` slots, twiddle tables, kernel constants.
@@ -692,20 +700,10 @@ static int setup_device(Ctx& c, int device, size_t slots) {
 
   c.pc = make_consts(c.pl);
   for (int k = 0; k < KMAX; ++k) {
-    c.nbase.p[k] = c.cbase.p[k] = c.pc.p[k];
-    c.nbase.q[k] = c.cbase.q[k] = c.pc.q[k];
+    c.nbase.p[k] = c.pc.p[k];
+    c.nbase.q[k] = c.pc.q[k];
     c.nbase.sc[k] = c.pc.sq_scale[k];
     c.nbase.scq[k] = c.pc.sq_scale_q[k];
-    c.cbase.c32[k] = c.pc.c32[k];
-    c.cbase.c32q[k] = c.pc.c32q[k];
-    c.cbase.c64[k] = c.pc.c64[k];
-    c.cbase.c64q[k] = c.pc.c64q[k];
-    c.cbase.c32n[k] = c.pc.p[k] - c.pc.c32[k];
-    c.cbase.c32nq[k] = c.cbase.c32n[k] * c.pc.q[k];
-    for (int j = 0; j < KMAX; ++j) {
-      c.cbase.ginv[k][j] = c.pc.ginv[k][j];
-      c.cbase.ginvq[k][j] = c.pc.ginvq[k][j];
-    }
   }
   c.nbase.LN = c.cbase.LN = c.pl.LN;
   c.cbase.cbits = c.pl.cbits();
@@ -756,7 +754,7 @@ static int load_slot_consts(Ctx& c, size_t slot, const mpz_t N, const mpz_t Ninv
   }
   mpz_t R, t;
   mpz_inits(R, t, nullptr);
-  plan_R(c.pl, R);
+  plan_R(slot_cbits(c, slot), c.pl.LN, R);
   mpz_mod(t, R, N);
   if (mpz_odd_p(t)) mpz_add(t, t, N);
   mpz_fdiv_q_2exp(t, t, 1);
@@ -1217,6 +1215,13 @@ int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int 
   int rc;
   if ((rc = setup_device(c, device, slots))) return rc;
   c.slotN.resize(slots);
+  c.slot_cb.assign(slots, c.pl.cbits());
+  if (hipMalloc(&c.cbits_dev, slots * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "per-slot digit widths");
+  {
+    std::vector<uint32_t> cb(slots, uint32_t(c.pl.cbits()));
+    HIPCHK(hipMemcpy(c.cbits_dev, cb.data(), slots * 4, hipMemcpyHostToDevice));
+  }
+  c.cbase.cbits_slot = c.cbits_dev;
   Stage& st = c.stage;
   st.cap = 2 * slots;
   if (hipMalloc(&st.dev, NSTAGE * st.cap * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "staging");
@@ -1245,11 +1250,13 @@ int annie_hip_batch_load(annie_hip_t* h, size_t slot, const uint64_t* N, size_t 
   limbs_to_mpz(m->v, N, nlimbs);
   if (mpz_even_p(m->v) || mpz_cmp_ui(m->v, 3) < 0) return fail(ANNIE_ERR_ARG, "N must be odd and >= 3");
   const Plan pl = plan_for(m->v, a);
-  if (pl.K != c.pl.K || pl.pck != c.pl.pck || pl.LN != c.pl.LN || pl.lbase != c.pl.lbase)
+  // Same K, pck and LN; lbase may differ within the batch's digit-width
+  // class (pck lbase >> 5, the carry kernel's limb shift).
+  if (pl.K != c.pl.K || pl.pck != c.pl.pck || pl.LN != c.pl.LN || (pl.cbits() >> 5) != (c.pl.cbits() >> 5))
     return fail(ANNIE_ERR_SIZE, "N belongs to another length class");
   mpz_t R, t;
   mpz_inits(R, t, nullptr);
-  plan_R(c.pl, R);
+  plan_R(pl, R);
   if (mpz_invert(t, m->v, R) == 0) {
     mpz_gcd(t, m->v, R);
     const int r = mpz_to_limbs(t, factor, cap, nout);
@@ -1258,6 +1265,10 @@ int annie_hip_batch_load(annie_hip_t* h, size_t slot, const uint64_t* N, size_t 
     return r == ANNIE_OK ? ANNIE_INIT_FAILED : r;
   }
   HIPCHK(hipSetDevice(c.device));
+  c.slot_cb[slot] = pl.cbits();
+  const uint32_t cb = uint32_t(pl.cbits());
+  HIPCHK(hipMemcpyAsync(c.cbits_dev + slot, &cb, 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipStreamSynchronize(c.stream));
   const int rc = load_slot_consts(c, slot, m->v, t);
   mpz_clears(R, t, nullptr);
   if (rc) return rc;

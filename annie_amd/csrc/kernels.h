@@ -71,12 +71,11 @@ struct CarryArgs {
   int njobs;
   const uint32_t* a_slot;  // per-position multiplier (replaces job.a) or null
   const uint32_t* snap;    // per-position flags (SLOT_SNAP): also write job.dst2, or null
-  uint32_t p[KMAX], q[KMAX];
-  uint32_t c32[KMAX], c32q[KMAX], c64[KMAX], c64q[KMAX];
-  uint32_t c32n[KMAX], c32nq[KMAX];  // p - c32 (2^64 mod p negated) and its product with p^-1
-  uint32_t ginv[KMAX][KMAX], ginvq[KMAX][KMAX];
   int LN;
   int cbits;  // bits per coefficient (pck * lbase)
+  // Batches whose slots differ in lbase: per-slot bits per coefficient
+  // (indexed by slot id), replacing cbits; all with the same cbits >> 5.
+  const uint32_t* cbits_slot;
 };
 
 // A run of batched squarings (one workgroup per listed slot): for slot list
@@ -86,7 +85,7 @@ struct CarryArgs {
 // + s * slot_stride words).
 struct RunArgs {
   NttArgs nt;            // transform constants (p, q, sc, scq, LN)
-  CarryArgs ca;          // CRT and carry constants (p, q, Garner inverses, cbits)
+  CarryArgs ca;          // carry parameters (LN, cbits); the prime constants are literals (PLIT)
   const uint32_t* slot_ids;
   const uint32_t* mult;
   size_t slot_stride;

@@ -749,8 +749,8 @@ __device__ __forceinline__ uint32_t digit_residue_wide(const uint32_t (&d)[DIGIT
 // K residues of CARRY_L digits, written as uint4 per plane at dst + i0.
 template <int K>
 __device__ __forceinline__ void store_residues(const uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], const CarryArgs& A,
-                                               uint32_t* dst, size_t n, size_t i0) {
-  if (A.cbits <= 64) {
+                                               uint32_t* dst, size_t n, size_t i0, int cbits) {
+  if (cbits <= 64) {
     // |d| <= 2^(cbits-1) + 1 < 2^64: the low 64 bits give the magnitude.
     uint32_t m0[CARRY_L], m1[CARRY_L];
     bool neg[CARRY_L];
@@ -929,7 +929,7 @@ __device__ int number_sign(const CarryJob& J, const CarryArgs& A, size_t n, uint
 // planes (and the second destination of a flagged slot).
 template <int K>
 __device__ __forceinline__ void carry_store(const CarryJob& J, const CarryArgs& A, size_t n, size_t b,
-                                            const uint32_t (&dig)[CARRY_L][DIGIT_LIMBS]) {
+                                            const uint32_t (&dig)[CARRY_L][DIGIT_LIMBS], int cbits) {
   const size_t i0 = b * CARRY_L;
   if (J.raw_out) {
 #pragma unroll
@@ -942,8 +942,8 @@ __device__ __forceinline__ void carry_store(const CarryJob& J, const CarryArgs& 
       *reinterpret_cast<uint4*>(J.dst + size_t(l) * n + i0) = v;
     }
   } else {
-    store_residues<K>(dig, A, J.dst, n, i0);
-    if (J.dst2 != nullptr) store_residues<K>(dig, A, J.dst2, n, i0);
+    store_residues<K>(dig, A, J.dst, n, i0, cbits);
+    if (J.dst2 != nullptr) store_residues<K>(dig, A, J.dst2, n, i0, cbits);
   }
 }
 
@@ -958,6 +958,7 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
   __shared__ uint32_t cl[L][CARRY_WG];
   const uint32_t nj = uint32_t(A.njobs);
   CarryJob J = A.job[blockIdx.y % nj];
+  int cbits = A.cbits;
   {
     const size_t so = slot_offset(A.slot_ids, A.slot_stride, blockIdx.y / nj);
     J.src += so;
@@ -965,11 +966,12 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     if (J.sub != nullptr) J.sub += so;
     if (A.a_slot != nullptr) J.a = A.a_slot[blockIdx.y / nj] & ~SLOT_SNAP;
     if (J.dst2 != nullptr) J.dst2 = (A.snap != nullptr && (A.snap[blockIdx.y / nj] & SLOT_SNAP)) ? J.dst2 + so : nullptr;
+    if (A.cbits_slot != nullptr) cbits = int(A.cbits_slot[A.slot_ids ? A.slot_ids[blockIdx.y / nj] : 0u]);
   }
   const size_t n = size_t(1) << A.LN;
   const size_t nb = n / CARRY_L;
   const uint32_t t = threadIdx.x;
-  const uint32_t s = uint32_t(A.cbits) & 31u;
+  const uint32_t s = uint32_t(cbits) & 31u;
   const size_t g = size_t(blockIdx.x) * (CARRY_WG - 1) + t;  // owned block + 1
   const size_t b = (g == 0) ? nb - 1 : g - 1;
   const bool active = g <= nb;
@@ -995,11 +997,11 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     // R = B^n - 1 to a negative value; then m' + 2.  (mod Q needs nothing: the
     // result is already the unique member of its class in (-B^n/2, B^n/2).)
     const int sg = number_sign<K, L, QS>(J, A, n, s);
-    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits);
+    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], cbits);
     if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
   }
 
-  carry_store<K>(J, A, n, b, dig);
+  carry_store<K>(J, A, n, b, dig, cbits);
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,10 +1042,10 @@ struct RunGeo {
 // `top`), and a third pass adjusts the end blocks as k_carry does and writes
 // the residues.
 template <int K, int QS, int NBT>
-__device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, size_t n, uint32_t T, uint32_t* cl,
-                                         int* top) {
+__device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, int cbits, size_t n, uint32_t T,
+                                         uint32_t* cl, int* top) {
   constexpr int L = K + 1;
-  const uint32_t s = uint32_t(A.cbits) & 31u;
+  const uint32_t s = uint32_t(cbits) & 31u;
   const size_t nb = n / CARRY_L;
   const uint32_t t = threadIdx.x;
   CarryJob raw = J;
@@ -1064,7 +1066,7 @@ __device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, 
     const size_t b = size_t(j) * T + t;
     uint32_t dig[CARRY_L][DIGIT_LIMBS], fo[L];
     carry_block_local<K, L, QS>(J, A, n, b, s, dig, fo);
-    carry_store<K>(raw, A, n, b, dig);
+    carry_store<K>(raw, A, n, b, dig, cbits);
 #pragma unroll
     for (int i = 0; i < L; ++i) cl[size_t(i) * nb + b] = fo[i];
   }
@@ -1088,9 +1090,9 @@ __device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, 
           break;
         }
       }
-      carry_store<K>(raw, A, n, b, dig);
+      carry_store<K>(raw, A, n, b, dig, cbits);
     } else {
-      carry_store<K>(J, A, n, b, dig);
+      carry_store<K>(J, A, n, b, dig, cbits);
     }
   }
   if (J.canon != 1) return;
@@ -1102,9 +1104,9 @@ __device__ __forceinline__ void wg_carry(const CarryJob& J, const CarryArgs& A, 
     const size_t b = size_t(j) * T + t;
     uint32_t dig[CARRY_L][DIGIT_LIMBS];
     load_digits(b, dig);
-    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], A.cbits);
+    if (sg < 0 && b == nb - 1) digit_add_base(dig[CARRY_L - 1], cbits);
     if (b == 0) digit_add(dig[0], ((sg < 0) ? -1 : 0) + 2);
-    carry_store<K>(J, A, n, b, dig);
+    carry_store<K>(J, A, n, b, dig, cbits);
   }
 }
 
@@ -1117,7 +1119,9 @@ __global__ void __launch_bounds__(RunGeo<M>::T, RunGeo<M>::WAVES) k_sqr_run(cons
   constexpr uint32_t S = 1u << M, Tl = RunGeo<M>::TL;
   constexpr int LINES = RunGeo<M>::LINES;
   const size_t n = S;
-  const size_t so = size_t(R.slot_ids[blockIdx.x]) * R.slot_stride;
+  const uint32_t slot = R.slot_ids[blockIdx.x];
+  const size_t so = size_t(slot) * R.slot_stride;
+  const int cbits = R.ca.cbits_slot ? int(R.ca.cbits_slot[slot]) : R.ca.cbits;
   const uint32_t line = threadIdx.x / Tl, tg = threadIdx.x % Tl;
   uint32_t* X = lds + line * (S + (S >> 5));
   const uint32_t* mult = R.mult + size_t(blockIdx.x) * size_t(R.steps);
@@ -1148,13 +1152,13 @@ __global__ void __launch_bounds__(RunGeo<M>::T, RunGeo<M>::WAVES) k_sqr_run(cons
     c.src = R.wr + so;
     c.dst = R.tr + so;
     c.a = a;
-    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, cbits, n, RunGeo<M>::T, lds, &top);
     __syncthreads();
     c.src = R.wq + so;
     c.dst = R.tq + so;
     c.nega = 1;
     c.raw_out = 1;
-    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, cbits, n, RunGeo<M>::T, lds, &top);
     __syncthreads();
     stage(std::integral_constant<int, ROW_MULC>(), m_job, K);
     c = CarryJob{};
@@ -1162,7 +1166,7 @@ __global__ void __launch_bounds__(RunGeo<M>::T, RunGeo<M>::WAVES) k_sqr_run(cons
     c.dst = R.tr + so;
     c.a = 1u;
     c.canon = 1;
-    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, cbits, n, RunGeo<M>::T, lds, &top);
     __syncthreads();
     stage(std::integral_constant<int, ROW_MULC>(), mn_job, K);
     c = CarryJob{};
@@ -1172,7 +1176,7 @@ __global__ void __launch_bounds__(RunGeo<M>::T, RunGeo<M>::WAVES) k_sqr_run(cons
     c.a = 1u;
     c.nega = 1;
     c.dst2 = (m & SLOT_SNAP) ? R.snap + so : nullptr;
-    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, n, RunGeo<M>::T, lds, &top);
+    wg_carry<K, QS, RunGeo<M>::NBT>(c, R.ca, cbits, n, RunGeo<M>::T, lds, &top);
     __syncthreads();
   }
 }

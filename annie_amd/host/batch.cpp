@@ -12,8 +12,9 @@
 // squares every test of the class.  A test that finishes frees its slot and
 // the next waiting test of its class is admitted at once (continuous
 // batching); tests that arrive later (serving) wait until their arrival time.
-// Classes are keyed by the plan (K, pck, LN, lbase) the reference's size
-// selection gives for (N, a), src/transformGPU.h:798-857.
+// Classes are keyed by the plan (K, pck, LN) the reference's size selection
+// gives for (N, a), src/transformGPU.h:798-857, and the digit-width class of
+// its lbase (pck lbase >> 5): plans that differ only in lbase share a batch.
 #include <gmp.h>
 
 #include <algorithm>
@@ -27,6 +28,8 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <mutex>
+#include <atomic>
 #include <tuple>
 #include <vector>
 
@@ -46,7 +49,9 @@ struct Big {
   Big& operator=(const Big&) = delete;
 };
 
-typedef std::tuple<int, int, int, int> PlanKey;  // K, pck, LN, lbase
+// K, pck, LN and the digit-width class pck lbase >> 5: moduli whose plans
+// differ only in lbase within a class share one engine batch.
+typedef std::tuple<int, int, int, int> PlanKey;
 
 // A batch of slots of one plan.  Operations act on the listed slots.
 class BatchEngine {
@@ -77,11 +82,13 @@ class HipBatch : public BatchEngine {
   }
 
  public:
-  HipBatch(const PlanKey& k, size_t slots, int device, bool graphs) {
-    const int K = std::get<0>(k), pck = std::get<1>(k), LN = std::get<2>(k), lbase = std::get<3>(k);
+  // lbase: that of one of the class's moduli.
+  HipBatch(const PlanKey& k, int lbase, size_t slots, int device, bool graphs) {
+    const int K = std::get<0>(k), pck = std::get<1>(k), LN = std::get<2>(k);
     check(annie_hip_batch_create(K, pck, LN, lbase, slots, device, &h_), "batch create");
     if (graphs) check(annie_hip_batch_set_graphs(h_, 1), "graphs");
-    stride_ = ((size_t(pck) * size_t(lbase)) << LN) / 64 + 2;
+    // limbs of a value below B^n with the widest digit of the class
+    stride_ = ((size_t(32) * size_t(std::get<3>(k) + 1)) << LN) / 64 + 2;
   }
   ~HipBatch() override { annie_hip_destroy(h_); }
   int load(size_t slot, const mpz_t N, int32_t a, mpz_t factor) override {
@@ -343,6 +350,7 @@ struct Job {
 
 struct Class {
   PlanKey key;
+  int lbase = 0;               // lbase of one of the class's moduli
   std::unique_ptr<BatchEngine> eng;
   size_t slots = 0;
   std::vector<Job*> slot;      // occupant per slot (null: free)
@@ -411,9 +419,10 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
       std::snprintf(r.message, sizeof(r.message), "Error: no transform of this size.");
       continue;
     }
-    j->key = PlanKey(K, pck, LN, lbase);
+    j->key = PlanKey(K, pck, LN, (pck * lbase) >> 5);
     Class& c = classes[j->key];
     c.key = j->key;
+    c.lbase = lbase;
     c.remaining++;
     jobs.push_back(std::move(j));
   }
@@ -439,188 +448,228 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
     c.slots = std::max<size_t>(s, 1);
   }
 
-  size_t done = 0;
-  const size_t total = jobs.size();
-  double squarings = 0, setup = 0;
+  // Host workers: each drives its own classes (class i to worker i mod W,
+  // W = min(classes, host_threads)).  A class has its own batch and stream,
+  // so a worker waits only for its own classes' work (a get() synchronises
+  // one stream), and the launches of different classes run side by side on
+  // the GPU instead of taking turns behind one host thread.
   const uint64_t flush_ticks = o.flush_ticks > 0 ? uint64_t(o.flush_ticks) : FLUSH;
-  bool stopped = false;
-  std::vector<Big*> vals;
-  std::vector<std::unique_ptr<Big>> store;
-  // Per-tick operation lists, reused across ticks and classes (cleared, not
-  // reallocated, every tick).
-  std::vector<uint32_t> set1, sq, sqa, run_ids, run_mult;
-  std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
-  std::map<int, std::vector<uint32_t>> gt;
-  std::vector<uint8_t> handled;
-  Big none;
-  auto finish = [&](Class& c, size_t s) {
-    Job* j = c.slot[s];
-    annie_batch_result& r = out[j->idx];
-    TestMachine& m = *j->m;
-    r.finished = now();
-    r.iterations = m.iterations;
-    r.is_prp = m.is_prp;
-    r.res64 = m.res64;
-    r.gl_ok = m.gl_ok;
-    const std::string ns = short_name(j->N.v);
-    std::string msg;
-    if (m.gl_ok) {
-      r.status = 0;
-      msg = m.is_prp ? ns + " is a " + (m.euler ? "Euler" : "Fermat") + " probable prime, a = " + std::to_string(m.a) + "."
-                     : ns + " is composite, res64 = " + hex64(m.res64) + ", a = " + std::to_string(m.a) + ".";
-    } else {
-      r.status = 1;
-      msg = ns + ": validation failed!";
-    }
-    std::snprintf(r.message, sizeof(r.message), "%s", msg.c_str());
-    if (o.verbose) std::fprintf(stderr, "[%.3f s] %s\n", r.finished, msg.c_str());
-    c.slot[s] = nullptr;
-    c.active--;
-    c.remaining--;
-    j->m.reset();
-    ++done;
-  };
+  std::vector<Class*> order;
+  for (auto& kv : classes) order.push_back(&kv.second);
+  // Graph capture (graphs on) stays on one host thread: the runtime rejects
+  // allocation and synchronisation calls of other threads while a stream is
+  // being captured (seen as a failed test with several workers).
+  const size_t want_threads = o.graphs ? 1 : (o.host_threads > 0 ? size_t(o.host_threads) : 16);
+  const size_t nthreads = std::max<size_t>(1, std::min(order.size(), want_threads));
+  std::mutex io_mu;
+  std::atomic<bool> failed{false};
+  std::string fail_msg;
+  std::vector<double> squarings_w(nthreads, 0.0), setup_w(nthreads, 0.0);
 
-  while (done < total && !stopped) {
-    const double tnow = now();
-    if (o.time_limit > 0 && tnow - setup >= o.time_limit) break;
-    // Admission: free slots take arrived tests of their class.
-    for (auto& kv : classes) {
-      Class& c = kv.second;
-      while (!c.queue.empty() && (!o.realtime || c.queue.front()->arrival <= tnow)) {
-        const double ts = now();
-        if (!c.eng) {
-          if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
-          else c.eng.reset(new HipBatch(c.key, c.slots, o.device, o.graphs != 0));
-          c.slot.assign(c.slots, nullptr);
-        }
-        size_t s = 0;
-        while (s < c.slots && c.slot[s]) ++s;
-        if (s == c.slots) break;
-        Job* j = c.queue.front();
-        c.queue.pop_front();
-        annie_batch_result& r = out[j->idx];
-        r.admitted = now();
-        Big f;
-        const int lr = c.eng->load(s, j->N.v, j->a, f.v);
-        setup += now() - ts;
-        if (lr == 1) {  // src/annie.h:269-283
-          r.status = 3;
-          r.finished = now();
-          std::string fs(mpz_sizeinbase(f.v, 10) + 2, '\0');
-          mpz_get_str(&fs[0], 10, f.v);
-          std::snprintf(r.message, sizeof(r.message), "%s divides %s.", fs.c_str(), short_name(j->N.v).c_str());
-          c.remaining--;
-          ++done;
-          continue;
-        }
-        j->m = std::make_unique<TestMachine>(j->N.v, j->a, j->euler);
-        c.slot[s] = j;
-        c.active++;
+  auto worker = [&](size_t w) {
+    std::vector<Class*> mine;
+    for (size_t i = w; i < order.size(); i += nthreads) mine.push_back(order[i]);
+    double& squarings = squarings_w[w];
+    double& setup = setup_w[w];
+    std::vector<Big*> vals;
+    std::vector<std::unique_ptr<Big>> store;
+    // Per-tick operation lists, reused across ticks and classes (cleared, not
+    // reallocated, every tick).
+    std::vector<uint32_t> set1, sq, sqa, run_ids, run_mult;
+    std::map<std::pair<int, int>, std::vector<uint32_t>> cp, ml;
+    std::map<int, std::vector<uint32_t>> gt;
+    std::vector<uint8_t> handled;
+    Big none;
+    auto finish = [&](Class& c, size_t s) {
+      Job* j = c.slot[s];
+      annie_batch_result& r = out[j->idx];
+      TestMachine& m = *j->m;
+      r.finished = now();
+      r.iterations = m.iterations;
+      r.is_prp = m.is_prp;
+      r.res64 = m.res64;
+      r.gl_ok = m.gl_ok;
+      const std::string ns = short_name(j->N.v);
+      std::string msg;
+      if (m.gl_ok) {
+        r.status = 0;
+        msg = m.is_prp ? ns + " is a " + (m.euler ? "Euler" : "Fermat") + " probable prime, a = " + std::to_string(m.a) +
+                             "."
+                       : ns + " is composite, res64 = " + hex64(m.res64) + ", a = " + std::to_string(m.a) + ".";
+      } else {
+        r.status = 1;
+        msg = ns + ": validation failed!";
       }
-    }
-    // One tick per class with active tests.
-    bool any = false;
-    for (auto& kv : classes) {
-      Class& c = kv.second;
-      if (c.active == 0) continue;
-      any = true;
-      set1.clear();
-      sq.clear();
-      sqa.clear();
-      for (auto& g : cp) g.second.clear();
-      for (auto& g : ml) g.second.clear();
-      for (auto& g : gt) g.second.clear();
-      // Deferred Gerbicz-Li products of all tests are taken together every
-      // FLUSH ticks (a test forces its own earlier if it must).  In run
-      // mode a tick is many squarings long: pending products are taken at
-      // every tick, and each test whose next operation is a squaring takes
-      // its following squarings (up to run_steps, ending at one that takes
-      // a Gerbicz-Li snapshot or before any other operation) in one launch.
-      const bool run_mode = o.run_steps > 0 && c.active >= std::max<size_t>(size_t(o.run_min_active), 1) &&
-                            c.eng->can_run();
-      const bool flush = run_mode || (++c.ticks % flush_ticks) == 0;
-      const size_t steps = size_t(std::max(o.run_steps, 1));
-      run_ids.clear();
-      run_mult.clear();
-      handled.assign(c.slots, 0);
-      for (size_t s = 0; s < c.slots; ++s) {
-        if (!c.slot[s]) continue;
-        TestMachine& m = *c.slot[s]->m;
-        const Op op = m.op(flush);
-        const uint32_t id = uint32_t(s);
-        if (run_mode && op.kind == OP_SQR) {
-          const size_t row = run_mult.size();
-          run_ids.push_back(id);
-          run_mult.resize(row + steps, 0u);
-          run_mult[row] = op.a;
-          m.advance(none.v);
-          size_t k = 1;
-          if (!(op.a & SNAP)) {
-            while (k < steps) {
-              const Op nx = m.op(false);
-              if (nx.kind != OP_SQR) break;
-              run_mult[row + k++] = nx.a;
-              m.advance(none.v);
-              if (nx.a & SNAP) break;
-            }
+      std::snprintf(r.message, sizeof(r.message), "%s", msg.c_str());
+      if (o.verbose) {
+        std::lock_guard<std::mutex> lock(io_mu);
+        std::fprintf(stderr, "[%.3f s] %s\n", r.finished, msg.c_str());
+      }
+      c.slot[s] = nullptr;
+      c.active--;
+      c.remaining--;
+      j->m.reset();
+    };
+
+    while (!failed.load(std::memory_order_relaxed)) {
+      const double tnow = now();
+      if (o.time_limit > 0 && tnow - setup >= o.time_limit) break;
+      // Admission: free slots take arrived tests of their class.
+      for (Class* cl : mine) {
+        Class& c = *cl;
+        while (!c.queue.empty() && (!o.realtime || c.queue.front()->arrival <= tnow)) {
+          const double ts = now();
+          if (!c.eng) {
+            if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
+            else c.eng.reset(new HipBatch(c.key, c.lbase, c.slots, o.device, o.graphs != 0));
+            c.slot.assign(c.slots, nullptr);
           }
-          squarings += double(k);
-          handled[s] = 1;
-          continue;
-        }
-        switch (op.kind) {
-          case OP_SET1: set1.push_back(id); break;
-          case OP_COPY: cp[{op.dst, op.src}].push_back(id); break;
-          case OP_SQR:
-            sq.push_back(id);
-            sqa.push_back(op.a);
-            break;
-          case OP_MUL: ml[{op.dst, op.src}].push_back(id); break;
-          case OP_GET: gt[op.dst].push_back(id); break;
-          default: break;
-        }
-      }
-      if (!set1.empty()) c.eng->set1(set1);
-      for (auto& g : cp)
-        if (!g.second.empty()) c.eng->copy(g.first.first, g.first.second, g.second);
-      if (!sq.empty()) {
-        c.eng->square_mul(sq, sqa);
-        squarings += double(sq.size());
-      }
-      if (!run_ids.empty()) c.eng->square_run(run_ids, run_mult, steps);
-      for (auto& g : ml)
-        if (!g.second.empty()) c.eng->mul(g.first.first, g.first.second, g.second);
-      for (auto& g : gt) {
-        if (g.second.empty()) continue;
-        store.clear();
-        vals.clear();
-        for (size_t i = 0; i < g.second.size(); ++i) {
-          store.push_back(std::make_unique<Big>());
-          vals.push_back(store.back().get());
-        }
-        c.eng->get(g.first, g.second, vals);
-        for (size_t i = 0; i < g.second.size(); ++i) {
-          handled[g.second[i]] = 1;
-          c.slot[g.second[i]]->m->advance(vals[i]->v);
-          if (c.slot[g.second[i]]->m->done()) finish(c, g.second[i]);
+          size_t s = 0;
+          while (s < c.slots && c.slot[s]) ++s;
+          if (s == c.slots) break;
+          Job* j = c.queue.front();
+          c.queue.pop_front();
+          annie_batch_result& r = out[j->idx];
+          r.admitted = now();
+          Big f;
+          const int lr = c.eng->load(s, j->N.v, j->a, f.v);
+          setup += now() - ts;
+          if (lr == 1) {  // src/annie.h:269-283
+            r.status = 3;
+            r.finished = now();
+            std::string fs(mpz_sizeinbase(f.v, 10) + 2, '\0');
+            mpz_get_str(&fs[0], 10, f.v);
+            std::snprintf(r.message, sizeof(r.message), "%s divides %s.", fs.c_str(), short_name(j->N.v).c_str());
+            c.remaining--;
+            continue;
+          }
+          j->m = std::make_unique<TestMachine>(j->N.v, j->a, j->euler);
+          c.slot[s] = j;
+          c.active++;
         }
       }
-      for (size_t s = 0; s < c.slots; ++s) {
-        if (!c.slot[s] || handled[s]) continue;
-        c.slot[s]->m->advance(none.v);
-        if (c.slot[s]->m->done()) finish(c, s);
+      // One tick per class with active tests.
+      bool any = false;
+      for (Class* cl : mine) {
+        Class& c = *cl;
+        if (c.active == 0) continue;
+        any = true;
+        set1.clear();
+        sq.clear();
+        sqa.clear();
+        for (auto& g : cp) g.second.clear();
+        for (auto& g : ml) g.second.clear();
+        for (auto& g : gt) g.second.clear();
+        // Deferred Gerbicz-Li products of all tests are taken together every
+        // FLUSH ticks (a test forces its own earlier if it must).  In run
+        // mode a tick is many squarings long: pending products are taken at
+        // every tick, and each test whose next operation is a squaring takes
+        // its following squarings (up to run_steps, ending at one that takes
+        // a Gerbicz-Li snapshot or before any other operation) in one launch.
+        const bool run_mode = o.run_steps > 0 && c.active >= std::max<size_t>(size_t(o.run_min_active), 1) &&
+                              c.eng->can_run();
+        const bool flush = run_mode || (++c.ticks % flush_ticks) == 0;
+        const size_t steps = size_t(std::max(o.run_steps, 1));
+        run_ids.clear();
+        run_mult.clear();
+        handled.assign(c.slots, 0);
+        for (size_t s = 0; s < c.slots; ++s) {
+          if (!c.slot[s]) continue;
+          TestMachine& m = *c.slot[s]->m;
+          const Op op = m.op(flush);
+          const uint32_t id = uint32_t(s);
+          if (run_mode && op.kind == OP_SQR) {
+            const size_t row = run_mult.size();
+            run_ids.push_back(id);
+            run_mult.resize(row + steps, 0u);
+            run_mult[row] = op.a;
+            m.advance(none.v);
+            size_t k = 1;
+            if (!(op.a & SNAP)) {
+              while (k < steps) {
+                const Op nx = m.op(false);
+                if (nx.kind != OP_SQR) break;
+                run_mult[row + k++] = nx.a;
+                m.advance(none.v);
+                if (nx.a & SNAP) break;
+              }
+            }
+            squarings += double(k);
+            handled[s] = 1;
+            continue;
+          }
+          switch (op.kind) {
+            case OP_SET1: set1.push_back(id); break;
+            case OP_COPY: cp[{op.dst, op.src}].push_back(id); break;
+            case OP_SQR:
+              sq.push_back(id);
+              sqa.push_back(op.a);
+              break;
+            case OP_MUL: ml[{op.dst, op.src}].push_back(id); break;
+            case OP_GET: gt[op.dst].push_back(id); break;
+            default: break;
+          }
+        }
+        if (!set1.empty()) c.eng->set1(set1);
+        for (auto& g : cp)
+          if (!g.second.empty()) c.eng->copy(g.first.first, g.first.second, g.second);
+        if (!sq.empty()) {
+          c.eng->square_mul(sq, sqa);
+          squarings += double(sq.size());
+        }
+        if (!run_ids.empty()) c.eng->square_run(run_ids, run_mult, steps);
+        for (auto& g : ml)
+          if (!g.second.empty()) c.eng->mul(g.first.first, g.first.second, g.second);
+        for (auto& g : gt) {
+          if (g.second.empty()) continue;
+          store.clear();
+          vals.clear();
+          for (size_t i = 0; i < g.second.size(); ++i) {
+            store.push_back(std::make_unique<Big>());
+            vals.push_back(store.back().get());
+          }
+          c.eng->get(g.first, g.second, vals);
+          for (size_t i = 0; i < g.second.size(); ++i) {
+            handled[g.second[i]] = 1;
+            c.slot[g.second[i]]->m->advance(vals[i]->v);
+            if (c.slot[g.second[i]]->m->done()) finish(c, g.second[i]);
+          }
+        }
+        for (size_t s = 0; s < c.slots; ++s) {
+          if (!c.slot[s] || handled[s]) continue;
+          c.slot[s]->m->advance(none.v);
+          if (c.slot[s]->m->done()) finish(c, s);
+        }
+      }
+      if (!any) {
+        // Nothing running: wait for the next arrival of one of these classes.
+        double next = -1;
+        for (Class* cl : mine)
+          if (!cl->queue.empty())
+            next = (next < 0) ? cl->queue.front()->arrival : std::min(next, cl->queue.front()->arrival);
+        if (next < 0) break;
+        const double wait = next - now();
+        if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
       }
     }
-    if (!any) {
-      // Nothing running: wait for the next arrival.
-      double next = -1;
-      for (auto& kv : classes)
-        if (!kv.second.queue.empty()) next = (next < 0) ? kv.second.queue.front()->arrival : std::min(next, kv.second.queue.front()->arrival);
-      if (next < 0) break;
-      const double wait = next - now();
-      if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+  };
+  auto guarded = [&](size_t w) {
+    try {
+      worker(w);
+    } catch (const std::exception& ex) {
+      std::lock_guard<std::mutex> lock(io_mu);
+      if (!failed.exchange(true)) fail_msg = ex.what();
     }
+  };
+  std::vector<std::thread> pool;
+  for (size_t w = 1; w < nthreads; ++w) pool.emplace_back(guarded, w);
+  guarded(0);
+  for (auto& t : pool) t.join();
+  if (failed) throw std::runtime_error(fail_msg);
+  double squarings = 0, setup = 0;
+  for (size_t w = 0; w < nthreads; ++w) {
+    squarings += squarings_w[w];
+    setup = std::max(setup, setup_w[w]);
   }
   // Tests cut off by the time limit.
   for (auto& j : jobs) {

@@ -361,7 +361,7 @@ def bench_proth_batch(args):
     progress(f"{len(tests)} Proth tests in {len(classes)} length classes; batched run (run steps {args.run_steps})")
     batch = [{"N": t["N"], "is_euler": True} for t in tests]
     res, st = annie_amd.batch_run(batch, engine="gpu", max_slots=args.max_slots, run_steps=args.run_steps,
-                                  run_min_active=args.run_min_active)
+                                  run_min_active=args.run_min_active, host_threads=args.host_threads, verbose=True)
     wall = st["seconds"]
     done = sum(r["status"] in ("success", "factor") for r in res)
     bad = [r for r in res if r["status"] not in ("success", "factor")]
@@ -431,8 +431,9 @@ def bench_proth_batch(args):
         "plain_note": "plain = one test at a time on the plain GPU engine, per-class measured rate x work",
         "dtype": "u32", "data": "synthetic", "verified": verified,
         "config": {"workload": "4,096 Proth k*2^n+1, k odd in [3, 2^20), n log-uniform [2e4, 4e5], seed 4, Euler test",
-                   "tests": len(tests), "classes": len(classes), "run_steps": args.run_steps,
-                   "run_min_active": args.run_min_active, "max_slots": args.max_slots},
+                   "tests": len(tests), "classes": len(classes), "batches": st["classes"],
+                   "run_steps": args.run_steps, "run_min_active": args.run_min_active,
+                   "host_threads": args.host_threads, "max_slots": args.max_slots},
         "roofline": roof,
         "cpu_baseline": cpu,
         "classes": rows,
@@ -493,7 +494,8 @@ def bench_serve(args):
     lo, hi = 1e4, args.serve_max_bits
     rng = random.Random(50)
     sample = [serve_request(rng, lo, hi) for _ in range(200)]
-    kw = dict(max_slots=args.max_slots, run_steps=args.run_steps, run_min_active=args.run_min_active)
+    kw = dict(max_slots=args.max_slots, run_steps=args.run_steps, run_min_active=args.run_min_active,
+              host_threads=args.host_threads)
     _, st = annie_amd.batch_run(sample, engine="gpu", time_limit=args.calib_seconds, **kw)
     cap_sq = st["squarings"] / max(st["seconds"] - st["setup_seconds"], 1e-9)
     mean_sq = sum(protocol_squarings(t["N"], t.get("is_euler", False)) for t in sample) / len(sample)
@@ -546,6 +548,79 @@ def bench_serve(args):
                                f"b log-uniform in [1e4, {hi:.0e}] bits, seed 5", "serve_seconds": args.serve_seconds,
                    "drain_seconds": args.serve_drain_seconds, "run_steps": args.run_steps,
                    "run_min_active": args.run_min_active},
+    }
+
+
+def bench_fermat332k(args):
+    """The whole Fermat PRP test (base 3, Gerbicz-Li included) of the seeded
+    ~332,000-bit N on the GPU path (plan K = 4, 2^13), seconds per test, its
+    residue against the GMP fixture; the dominant kernel's live roofline over
+    a profile window of the same squaring; the reference's own CPU transform
+    (transformCPU, src/transformCPU.h, K = 5) timed on a bounded sample of the
+    same N's squarings on one core, extrapolated to the test's square_mul
+    count."""
+    import annie_amd
+    import oracle  # the CPU baseline leg and the fixture's N only
+    from tests.golden.make_config_fixtures import cfg1_N
+    N, cand = cfg1_N()
+    fx_path = os.path.join(ROOT, "tests", "golden", "config_res64.json")
+    fx = json.load(open(fx_path))["cfg1"] if os.path.exists(fx_path) else None
+    progress("GPU: whole Fermat test of the 332,000-bit N")
+    r = annie_amd.check(N, a=3, gmp_threshold_digits=1, min_ln=5)
+    if r["status"] != "success" or not r["gl_ok"]:
+        print(json.dumps({"error": r["message"]}), flush=True)
+        sys.exit(1)
+    verified = []
+    if fx and fx.get("candidate") == cand:
+        ok = (r["res64"], r["is_prp"]) == (fx["res64"], fx["is_prp"])
+        verified.append({"res64": f"{r['res64']:016X}", "match": ok, "against": "mpz_powm, " +
+                         os.path.relpath(fx_path, ROOT)})
+        if not ok:
+            print(json.dumps({"error": "GPU residue differs from the GMP fixture", "verified": verified}), flush=True)
+            sys.exit(1)
+    work = protocol_squarings(N, False)
+    # profile window on the plain engine: per-kernel HIP event pairs
+    T = annie_amd.Transform(N, 3, min_ln=5)
+    U = T.K * T.n * 4
+    bits = exponent_bits(N, 400)
+    T.set1()
+    T.square_mul_bits(bits[:100], 3)
+    T.profile_begin()
+    T.square_mul_bits(bits[100:400], 3)
+    prof = T.profile_end()
+    plan = (T.K, T.pck, T.LN, T.lbase)
+    T.close()
+    dom = max(prof, key=lambda k: prof[k][1])
+    cnt, tot = prof[dom]
+    achieved = alg_bytes(dom, U, plan[0]) / (tot / cnt * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        kind = "reference" if oracle.ref_available() else "port"
+        C = oracle.RefTransform(N, 3, 1) if kind == "reference" else oracle.NttTransform(N, 3)
+        C.set1()
+        done, t0 = 0, time.time()
+        while done < 3 or time.time() - t0 < args.cpu_seconds:
+            C.square_mul(3 if bits[done % len(bits)] else 1)
+            done += 1
+        dt = time.time() - t0
+        C.close()
+        cpu = {"value": round(dt / done * work, 1), "unit": "s/test", "cores": 1, "kind": kind,
+               "sample": f"{done} squarings of the same N ({dt:.1f} s) on one core with the reference's "
+                         f"transformCPU, times the test's {work} square_mul operations",
+               "ms_per_squaring": round(dt * 1e3 / done, 4), **host_info()}
+    return {
+        "metric": "s per Fermat PRP test, 332,000-bit N (Gerbicz-Li included)", "value": round(r["seconds"], 3),
+        "unit": "s/test", "n_gpus": 1, "higher_is_better": False, "dtype": "u32", "data": "synthetic",
+        "ms_per_squaring": round(r["seconds"] * 1e3 / work, 5), "square_muls": work, "verified": verified,
+        "config": {"workload": f"Fermat PRP test, seeded random odd 332,000-bit N (seed 1, candidate {cand}), base 3",
+                   "K": plan[0], "pck": plan[1], "LN": plan[2], "lbase": plan[3]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes(dom, U, plan[0]), "avg_launch_ms": round(tot / cnt, 5),
+                     "note": "a one-pass length: each launch is a few microseconds, the squaring is bound by "
+                             "launch latency, not by HBM or VALU"},
+        "kernels_ms_per_squaring": {k: round(v[1] / 300, 5) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+        "cpu_baseline": cpu,
     }
 
 
@@ -628,7 +703,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-baseline-worker", type=int, default=None, help=argparse.SUPPRESS)
-    ap.add_argument("--workload", choices=["squaring", "proth3m", "proth-batch", "serve"], default="squaring",
+    ap.add_argument("--workload", choices=["squaring", "fermat332k", "proth3m", "proth-batch", "serve"],
+                    default="squaring",
                     help="squaring: the ~40M-digit square_mul loop (default); proth3m: k*2^3,000,000+1; "
                          "proth-batch: 4,096 Proth tests, batched vs plain; serve: Poisson arrivals with "
                          "continuous batching")
@@ -638,6 +714,8 @@ def main():
     ap.add_argument("--run-steps", type=int, default=32, help="proth-batch/serve: squarings per test per launch")
     ap.add_argument("--run-min-active", type=int, default=4,
                     help="proth-batch/serve: running tests a class needs for squaring runs")
+    ap.add_argument("--host-threads", type=int, default=0,
+                    help="proth-batch/serve: host workers driving the classes (0 = the driver's default)")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-class-seconds", type=float, default=1.0)
     ap.add_argument("--serve-seconds", type=float, default=120.0)
@@ -657,7 +735,8 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
     if args.workload != "squaring":
-        line = {"proth3m": bench_proth3m, "proth-batch": bench_proth_batch, "serve": bench_serve}[args.workload](args)
+        line = {"fermat332k": bench_fermat332k, "proth3m": bench_proth3m, "proth-batch": bench_proth_batch,
+                "serve": bench_serve}[args.workload](args)
         if rank == 0:
             print(json.dumps(line), flush=True)
         return

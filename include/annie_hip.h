@@ -143,7 +143,9 @@ int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int 
  * digit buffers and slot-list staging) and bytes held once per batch (twiddle
  * tables), for admission under an HBM budget. */
 int annie_hip_batch_bytes(int K, int LN, size_t* per_slot, size_t* fixed);
-/* Loads modulus N (base a) into a slot: its plan must be the batch's.  Returns
+/* Loads modulus N (base a) into a slot: its plan must have the batch's K, pck
+ * and LN, and lbase with the same pck * lbase >> 5 (the batch's lbase or
+ * another of that digit-width class: each slot keeps its own).  Returns
  * ANNIE_INIT_FAILED with gcd(N, R) in `factor` when N is not invertible mod R
  * (src/transformGPU.h:652-661). */
 int annie_hip_batch_load(annie_hip_t* b, size_t slot, const uint64_t* N, size_t nlimbs, int32_t a, uint64_t* factor,

@@ -81,6 +81,8 @@ typedef struct {
                                  that many squarings per test in one launch (squaring runs, lengths
                                  2^9 .. 2^13); 0: one squaring per tick */
   int run_min_active;         /* running tests a class needs for squaring runs, 0 = 1 */
+  int host_threads;           /* host worker threads driving the classes (class i to worker i mod W),
+                                 0 = min(classes, 16); one when graphs is set */
 } annie_batch_opts;
 
 typedef struct {

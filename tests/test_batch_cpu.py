@@ -128,3 +128,17 @@ def test_hbm_budget_bounds_the_slots_of_each_class():
     _assert_matches(tests, res)
     admitted = sorted(r["admitted"] for r in res if r["status"] == "success")
     assert admitted[-1] > admitted[2]  # later tests waited for a free slot
+
+
+@pytest.mark.parametrize("host_threads", [1, 3, 16])
+def test_host_workers_give_the_same_results(host_threads):
+    """Classes driven by 1, 3 or 16 host worker threads: the same results
+    test by test, every square_mul accounted for."""
+    tests = _mix(13, 20)
+    base, st0 = annie_amd.batch_run(tests, engine="gmp", max_slots=4, host_threads=1)
+    res, st = annie_amd.batch_run(tests, engine="gmp", max_slots=4, host_threads=host_threads)
+    assert st["rc"] == 0 and st["squarings"] == st0["squarings"]
+    for a, b in zip(base, res):
+        assert (a["status"], a["res64"], a["is_prp"], a["gl_ok"], a["iterations"]) == \
+            (b["status"], b["res64"], b["is_prp"], b["gl_ok"], b["iterations"])
+    _assert_matches(tests, res)

@@ -52,12 +52,14 @@ def test_batch_fermat_mix_matches_powm(gpu):
         assert r["res64"] == oracle.powm(3, t["N"] - 1, t["N"]) & (2 ** 64 - 1)
 
 
-def test_graph_replays_match_launches(gpu):
+@pytest.mark.parametrize("host_threads", [1, 4])
+def test_graph_replays_match_launches(gpu, host_threads):
     tests = _proth_mix(45, 10, 1500, 5000)
-    a, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3)
-    b, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3, graphs=True)
+    a, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3, host_threads=host_threads)
+    b, _ = annie_amd.batch_run(tests, engine="gpu", max_slots=3, graphs=True, host_threads=host_threads)
     for x, y in zip(a, b):
-        assert (x["status"], x["res64"], x["is_prp"], x["gl_ok"]) == (y["status"], y["res64"], y["is_prp"], y["gl_ok"])
+        assert (x["status"], x["res64"], x["is_prp"], x["gl_ok"]) == (y["status"], y["res64"], y["is_prp"], y["gl_ok"]), \
+            (x["message"], y["message"])
 
 
 def test_same_result_alone_and_in_the_mix(gpu):

@@ -35,3 +35,31 @@ def test_quick_test_list_batched(gpu):
         print(f"case fibonacci({i}): {r['message'][-70:]}", flush=True)
         assert r["status"] == "success" and r["gl_ok"], r
         assert r["is_prp"] == (i not in COMPOSITE)
+
+
+# The rest of the QUICK_TEST list (src/main.cpp:163) above 400,000 bits, run
+# through the command-line front end (the expression evaluated by the CLI's
+# parser, the test on the GPU engine, src/main.cpp:246-264): each is a
+# Fermat probable prime with every Gerbicz-Li check passing.
+CLI_QUICK = [590041, 1285607, 2904353]
+
+
+@pytest.mark.parametrize("i", CLI_QUICK)
+def test_quick_test_list_cli(gpu, capsys, tmp_path, i):
+    import os
+    import subprocess
+    exe = os.path.join(annie_amd.LIBDIR, "annie")
+    expr = f"fibonacci({i})"
+    lines = []
+    with capsys.disabled():
+        print(f"case CLI {expr}", flush=True)
+        p = subprocess.Popen([exe, "-q", expr, "-w", str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                             text=True)
+        for ln in p.stdout:  # progress lines every 10 s keep the log moving
+            print("  " + ln.rstrip(), flush=True)
+            lines.append(ln)
+        rc = p.wait(timeout=600)
+    out = "".join(lines)
+    assert rc == 0, out[-2000:]
+    assert f"{expr} is a Fermat probable prime, a = 2." in out, out[-2000:]
+    assert "validation failed" not in out.lower()

@@ -16,7 +16,11 @@ pytestmark = pytest.mark.gpu
 
 # (bits, seed): single-pass lengths 2^5..2^13 and two-pass lengths 2^14..2^17.
 SIZES = [(2000, 1), (5000, 2), (20000, 3), (60000, 4), (150000, 5), (300000, 6), (500000, 7),
-         (1000000, 8), (2000000, 9), (4000000, 10)]
+         (1000000, 8), (2000000, 9), (4000000, 10), (332000, 12), (3000020, 11)]
+# Plans the reference selects at the sizes of the configurations, asserted
+# where they are the point of the case: the 332,000-bit Fermat test and the
+# Proth k*2^3,000,000+1 test run on K = 4 (one-pass 2^13, two-pass 2^16).
+PLAN_AT = {332000: (4, 2, 13, 28), 3000020: (4, 2, 16, 27)}
 
 
 def _flush(msg):
@@ -31,6 +35,8 @@ def test_square_mul_registers_match_oracle(gpu, bits, seed):
     t = annie_amd.Transform(N, a, min_ln=5)
     o = oracle.NttTransform(N, a)
     assert (t.K, t.pck, t.LN, t.lbase) == (o.K, o.pck, o.LN, o.lbase)
+    if bits in PLAN_AT:
+        assert (t.K, t.pck, t.LN, t.lbase) == PLAN_AT[bits]
     t.set1()
     o.set1()
     assert_same_words(t.read_register(0), o.read_register(0), "set1", t.n)

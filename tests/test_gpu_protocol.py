@@ -109,26 +109,61 @@ def test_factor_reported_when_gcd_with_R(gpu):
     assert r["message"].startswith(str(g))
 
 
-def test_prefix_at_40M_digits_matches_gmp_fixture(gpu):
-    """The benchmark's 132,900,000-bit N: residue after the first t loop
-    iterations == 3^(top t bits of N-1) mod N from mpz_powm (committed fixture)."""
+def test_gerbicz_li_and_checkpoint_at_40M_digits(gpu):
+    """The benchmark's 132,900,000-bit N over the exponent made of the top
+    10,000 bits of N - 1 (the timed depth): the hot loop with the Gerbicz-Li
+    products every B = 79 iterations (src/annie.h:319-341), a checkpoint
+    round trip of r0 and r1 through host bytes half-way (the device
+    registers are overwritten in between), then d(t+1) = d(t) result and the
+    verification a^(res - 2^B) d(t)^(2^B) a^(res mod 2^B) == d(t+1)
+    (src/annie.h:343-404); the residues after 100 and 1,000 iterations
+    against the GMP fixture (mpz_powm of the same prefixes)."""
     import bench
-    path = os.path.join(HERE, "golden", "cfg3_prefix.json")
-    fx = json.load(open(path))
+    fx = json.load(open(os.path.join(HERE, "golden", "cfg3_prefix.json")))
     N = bench.workload_N(skip=fx["candidate"])
-    t = annie_amd.Transform(N, fx["base"])
-    assert not t.init_failed()
-    assert (t.K, t.LN, t.lbase) == (5, 21, 16)
-    e = N - 1
-    done = 0
+    a = fx["base"]
+    t = annie_amd.Transform(N, a)
+    assert (t.K, t.LN) == (5, 21)
+    L = 10_000
+    e = (N - 1) >> ((N - 1).bit_length() - L)
+    B = oracle.gl_block(L)
+    assert B == 79
+    prefix = {x["iterations"]: x for x in fx["entries"]}
+    assert set(prefix) == {100, 1000}
+    nbytes = (N.bit_length() + 7) // 8
     t.set1()
-    for ent in sorted(fx["entries"], key=lambda d: d["iterations"]):
-        it = ent["iterations"]
-        bits = bench.exponent_bits(N, it)[done:]
-        t.square_mul_bits(bits, fx["base"])
-        done = it
-        v = t.get(0)
-        assert v & (2 ** 64 - 1) == ent["res64"], f"res64 after {it} iterations"
-        nbytes = (N.bit_length() + 7) // 8
-        assert hashlib.sha256(v.to_bytes(nbytes, "little")).hexdigest() == ent["sha256"]
+    t.copy(1, 0)
+    seg = []
+    for i in range(L - 1, -1, -1):  # the hot loop, squarings issued in segments
+        seg.append((e >> i) & 1)
+        done = L - i
+        gl = i % B == 0 and i // B != 0
+        if gl or done in (100, 1000, 5000):
+            t.square_mul_bits(seg, a)
+            seg = []
+        if gl:
+            t.mul(1, 0)
+        if done in prefix:  # a^(top `done` bits of e) mod N from mpz_powm
+            v = t.get(0)
+            assert v & (2 ** 64 - 1) == prefix[done]["res64"], f"res64 after {done} iterations"
+            assert hashlib.sha256(v.to_bytes(nbytes, "little")).hexdigest() == prefix[done]["sha256"]
+        if done == 5000:  # checkpoint round trip half-way
+            saved = t.save_checkpoint()
+            t.set1()
+            t.copy(1, 0)
+            assert t.read_checkpoint(saved)
+    t.square_mul_bits(seg, a)
+    t.copy(2, 0)
+    t.mul(2, 1)
+    res, ee = 0, e
+    while ee:
+        res += ee & ((1 << B) - 1)
+        ee >>= B
+    rs = res.bit_length()
+    t.set1()
+    t.square_mul_bits([(res >> j) & 1 for j in range(rs - 1, B - 1, -1)], a)
+    t.mul(0, 1)
+    t.square_mul_bits([(res >> j) & 1 if j < rs else 0 for j in range(B - 1, -1, -1)], a)
+    r1, r2 = t.get(0), t.get(2)
+    assert r1 == r2, "Gerbicz-Li check failed"
     t.close()

@@ -136,3 +136,59 @@ def test_driver_run_mode_matches_oracle(gpu):
         o = oracle.check(t["N"], is_euler=True)
         assert r["status"] == "success" and r["gl_ok"], r
         assert (r["res64"], r["is_prp"], r["a"]) == (o["res64"], o["is_prp"], o["a"])
+
+
+@pytest.mark.parametrize("K,LN,first,second", [(5, 11, (110750, 13, 17), (114750, 2, 18)),
+                                               (4, 13, (311500, 5, 27), (327750, 2, 28))])
+def test_batch_slots_of_two_lbase(gpu, K, LN, first, second):
+    """Moduli of one (K, LN) whose plans differ in lbase (the base changes the
+    CRT bound, src/transformGPU.h:825-850) share one batch, each slot with
+    its own digit width: squarings issued one by one and as runs give the
+    exact powers a_s^e mod N_s."""
+    rng = random.Random(K + LN)
+    Ns, bases = [], []
+    for bits, a, lbase in (first, second, first, second):
+        while True:
+            N = rng.getrandbits(bits) | (1 << (bits - 1)) | 1
+            p = annie_amd.select_plan(N, a)
+            if (p[0], p[2], p[3]) != (K, LN, lbase):
+                continue
+            t = annie_amd.Transform(N, a, min_ln=5)
+            ok = not t.init_failed()
+            t.close()
+            if ok:
+                break
+        Ns.append(N)
+        bases.append(a)
+    print(f"case mixed lbase K={K} LN={LN} lbase {first[2]}/{second[2]}", flush=True)
+    lib = annie_amd.load_library()
+    K0, pck, LN0, lb0 = annie_amd.select_plan(Ns[0], bases[0])
+    h = ctypes.c_void_p()
+    assert lib.annie_hip_batch_create(K0, pck, LN0, lb0, 4, 0, ctypes.byref(h)) == 0, annie_amd._err(lib)
+    try:
+        for s, (N, a) in enumerate(zip(Ns, bases)):
+            arr, n = annie_amd.int_to_limbs(N)
+            f = (ctypes.c_uint64 * (n + 1))()
+            nf = ctypes.c_size_t()
+            assert lib.annie_hip_batch_load(h, s, arr, n, a, f, n + 1, ctypes.byref(nf)) == 0, annie_amd._err(lib)
+        ids = list(range(4))
+        assert lib.annie_hip_batch_set1(h, (u32 * 4)(*ids), 4) == 0
+        exps = [0] * 4
+        for step in range(10):  # one by one
+            m = [bases[s] if rng.getrandbits(1) else 1 for s in ids]
+            for s in ids:
+                exps[s] = 2 * exps[s] + (1 if m[s] != 1 else 0)
+            assert lib.annie_hip_batch_square_mul(h, (u32 * 4)(*ids), (u32 * 4)(*m), 4) == 0
+        steps = 9
+        mult = []
+        for s in ids:  # one run each
+            row = [bases[s] if rng.getrandbits(1) else 1 for _ in range(steps)]
+            for x in row:
+                exps[s] = 2 * exps[s] + (1 if x != 1 else 0)
+            mult += row
+        assert lib.annie_hip_batch_square_run(h, (u32 * 4)(*ids), 4, (u32 * len(mult))(*mult), steps) == 0, \
+            annie_amd._err(lib)
+        got = _values(lib, h, 0, ids, Ns)
+        assert got == [pow(a, e, N) for a, e, N in zip(bases, exps, Ns)]
+    finally:
+        lib.annie_hip_destroy(h)
