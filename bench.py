@@ -702,6 +702,9 @@ def main():
     ap.add_argument("--gpu-warmup-seconds", type=float, default=1.0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fixture-check", action="store_true",
+                    help="squaring: skip the residue check against the GMP fixture and its untimed lead-in "
+                         "(profiler runs, where every dispatch is slow)")
     ap.add_argument("--cpu-baseline-worker", type=int, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--workload", choices=["squaring", "fermat332k", "proth3m", "proth-batch", "serve"],
                     default="squaring",
@@ -765,6 +768,8 @@ def main():
     # exactly that many iterations and the final residue is compared; longer
     # runs compare the residue after the fixture's shortest prefix, taken
     # before the warm-up.
+    if args.no_fixture_check:  # profiler runs: no lead-in, no residue check
+        fixture = {}
     longest = max(fixture) if fixture else 0
     if fixture and total <= longest:
         lead, pre = longest - total, 0
