@@ -382,9 +382,12 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
     mpz_import(j->N.v, tests[t].nlimbs, -1, 8, 0, 0, tests[t].N);
     j->euler = tests[t].is_euler != 0;
     j->arrival = tests[t].arrival;
-    const std::string ns = short_name(j->N.v);
-    // Decided without the loop (src/annie.h:222-247).
-    if (ns.size() <= 15) {
+    // Decided without the loop (src/annie.h:222-247).  The decimal string is
+    // only formed for numbers that small (or even): at 2e7 bits it takes
+    // seconds, and the messages of tests that run form it when they finish.
+    const bool tiny = mpz_sizeinbase(j->N.v, 10) <= 16 && short_name(j->N.v).size() <= 15;
+    if (tiny) {
+      const std::string ns = short_name(j->N.v);
       const int pp = mpz_probab_prime_p(j->N.v, 20);
       r.status = 4;
       r.is_prp = pp != 0;
@@ -393,7 +396,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
     }
     if (mpz_even_p(j->N.v)) {
       r.status = 4;
-      std::snprintf(r.message, sizeof(r.message), "%s is even.", ns.c_str());
+      std::snprintf(r.message, sizeof(r.message), "%s is even.", short_name(j->N.v).c_str());
       continue;
     }
     int32_t a = 2;  // src/annie.h:249-254
@@ -466,6 +469,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
   std::string fail_msg;
   std::vector<double> squarings_w(nthreads, 0.0), setup_w(nthreads, 0.0);
 
+  const double t_start = now();  // time_limit counts from here, set-up excluded
   auto worker = [&](size_t w) {
     std::vector<Class*> mine;
     for (size_t i = w; i < order.size(); i += nthreads) mine.push_back(order[i]);
@@ -513,7 +517,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
 
     while (!failed.load(std::memory_order_relaxed)) {
       const double tnow = now();
-      if (o.time_limit > 0 && tnow - setup >= o.time_limit) break;
+      if (o.time_limit > 0 && tnow - t_start - setup >= o.time_limit) break;
       // Admission: free slots take arrived tests of their class.
       for (Class* cl : mine) {
         Class& c = *cl;

@@ -496,7 +496,11 @@ def bench_serve(args):
     sample = [serve_request(rng, lo, hi) for _ in range(200)]
     kw = dict(max_slots=args.max_slots, run_steps=args.run_steps, run_min_active=args.run_min_active,
               host_threads=args.host_threads)
-    _, st = annie_amd.batch_run(sample, engine="gpu", time_limit=args.calib_seconds, **kw)
+    cres, st = annie_amd.batch_run(sample, engine="gpu", time_limit=args.calib_seconds, **kw)
+    errs = [r["message"] for r in cres if r["status"] == "error"]
+    if st["rc"] != 0 or errs or st["squarings"] == 0:
+        print(json.dumps({"error": "serving calibration failed", "stats": st, "errors": errs[:5]}), flush=True)
+        sys.exit(1)
     cap_sq = st["squarings"] / max(st["seconds"] - st["setup_seconds"], 1e-9)
     mean_sq = sum(protocol_squarings(t["N"], t.get("is_euler", False)) for t in sample) / len(sample)
     rate = 0.7 * cap_sq / mean_sq
