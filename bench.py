@@ -291,13 +291,16 @@ def cpu_class_rate(job):
     return {"rate": done / dt, "kind": "reference" if ref else "port"}
 
 
-def run_roofline(cls_tests, steps, repeats=3):
-    """Live roofline of the squaring-run kernel on one class: a batch of the
-    class's tests, `repeats` launches of `steps` squarings per slot timed with
-    HIP events around each launch (the engine's profile window).  Algorithmic
-    bytes of a single-pass squaring: 11 U (U = K n 4), the compulsory traffic
-    of the four convolutions' register sweeps (the registers stay
-    L2/MALL-resident at these lengths, so this is a bound, not HBM traffic)."""
+def batch_roofline(cls_tests, steps, repeats=3):
+    """Live roofline of the batched path on one class: a batch of the class's
+    tests timed with HIP events around each launch (the engine's profile
+    window) -- with steps > 0, `repeats` squaring-run launches of `steps`
+    squarings per slot; with steps = 0, `repeats` x 8 batched squarings
+    issued launch by launch, the dominant kernel reported.  Algorithmic bytes
+    of a single-pass squaring: 11 U (U = K n 4) -- per launch of the launched
+    path, alg_bytes() of the kernel times the slots -- the compulsory traffic
+    of the registers (L2/MALL-resident at these lengths, so a bound, not HBM
+    traffic)."""
     import ctypes
 
     import annie_amd
@@ -317,25 +320,38 @@ def run_roofline(cls_tests, steps, repeats=3):
         u32 = ctypes.c_uint32
         sl = (u32 * len(ids))(*ids)
         lib.annie_hip_batch_set1(h, sl, len(ids))
-        mult = (u32 * (len(ids) * steps))(*([3] * (len(ids) * steps)))
-        lib.annie_hip_batch_square_run(h, sl, len(ids), mult, steps)  # warm-up
+        if steps > 0:
+            mult = (u32 * (len(ids) * steps))(*([3] * (len(ids) * steps)))
+            launch = lambda: lib.annie_hip_batch_square_run(h, sl, len(ids), mult, steps)  # noqa: E731
+            per, sq = repeats, len(ids) * steps
+        else:
+            mult = (u32 * len(ids))(*([3] * len(ids)))
+            launch = lambda: lib.annie_hip_batch_square_mul(h, sl, mult, len(ids))  # noqa: E731
+            per, sq = repeats * 8, len(ids)
+        launch()  # warm-up
         lib.annie_hip_batch_sync(h)
         lib.annie_hip_profile_begin(h)
-        for _ in range(repeats):
-            lib.annie_hip_batch_square_run(h, sl, len(ids), mult, steps)
+        for _ in range(per):
+            launch()
         buf = ctypes.create_string_buffer(1 << 14)
         lib.annie_hip_profile_end(h, buf, 1 << 14)
     finally:
         lib.annie_hip_destroy(h)
-    name, cnt, tot = buf.value.decode().split()[:3]
-    avg_ms = float(tot) / int(cnt)
+    prof = {}
+    for ln in buf.value.decode().splitlines():
+        name, cnt, tot = ln.split()
+        prof[name] = (int(cnt), float(tot))
+    name = max(prof, key=lambda k: prof[k][1])
+    cnt, tot = prof[name]
+    avg_ms = tot / cnt
     U = K * (1 << LN) * 4
-    alg = len(ids) * steps * 11 * U
+    alg = len(ids) * (steps * 11 * U if steps > 0 else alg_bytes(name, U, K))
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    total_ms = sum(v[1] for v in prof.values())
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_launch": alg,
-            "avg_launch_ms": round(avg_ms, 4), "slots": len(ids), "steps": steps,
-            "squarings_per_s": round(len(ids) * steps / (avg_ms * 1e-3), 1)}
+            "avg_launch_ms": round(avg_ms, 4), "slots": len(ids), "plan": [K, pck, LN, lbase],
+            "squarings_per_s_in_window": round(per * sq / (total_ms * 1e-3), 1)}
 
 
 def bench_proth_batch(args):
@@ -421,7 +437,7 @@ def bench_proth_batch(args):
     # Live roofline of the run kernel on the class with the most square_mul work.
     big = max(range(len(rows)), key=lambda i: rows[i]["square_muls"])
     big = (rows[big]["K"], rows[big]["pck"], rows[big]["LN"], rows[big]["lbase"])
-    roof = run_roofline(classes[big][:256], args.run_steps) if args.run_steps > 0 else None
+    roof = batch_roofline(classes[big][:256], args.run_steps)
     return {
         "metric": "batched Proth tests/hour (4,096 k*2^n+1, n log-uniform in [2e4, 4e5]), measured completions",
         "value": round(tph_b, 1), "unit": "tests/hour", "n_gpus": 1, "higher_is_better": True,
@@ -718,7 +734,8 @@ def main():
     ap.add_argument("--tests", type=int, default=4096)
     ap.add_argument("--plain-steps", type=int, default=400)
     ap.add_argument("--max-slots", type=int, default=0)
-    ap.add_argument("--run-steps", type=int, default=32, help="proth-batch/serve: squarings per test per launch")
+    ap.add_argument("--run-steps", type=int, default=0,
+                    help="proth-batch/serve: squarings per test per launch (squaring runs; 0 = off)")
     ap.add_argument("--run-min-active", type=int, default=4,
                     help="proth-batch/serve: running tests a class needs for squaring runs")
     ap.add_argument("--host-threads", type=int, default=0,
