@@ -269,11 +269,15 @@ __device__ __forceinline__ uint32_t lpad(uint32_t i) { return i + (i >> 5); }
 // 16 elements per thread (LR = 4 levels per register round; 32 elements per
 // thread at rows of 2^11 measured no faster).
 // Grid: (rows / lines-per-block, K, jobs).  Block: lines * 2^M / 16 threads.
+// Rows of 2^11 (the 40M-digit length): five waves per SIMD, so that a one-job
+// pass over five planes (10,240 waves) is exactly two resident rounds and a
+// two-job pass four, instead of 2.5 and 5 at four waves (the half round left
+// half the chip idle at the end of every one-job pass).
 template <int M>
 struct RowGeo {
   static constexpr int LR = NTT_LOGR;
   static constexpr int THREADS = 512;
-  static constexpr int WAVES_PER_EU = 4;
+  static constexpr int WAVES_PER_EU = (M == 11) ? 5 : 4;
 };
 
 // One line of a row pass (the squaring runs below; the launched pass k_row
@@ -365,8 +369,11 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
   const int k = blockIdx.y;
   const uint32_t p = A.p[k];
   const size_t n = size_t(1) << A.LN;
-  const uint32_t line = threadIdx.x / Tl, tg = threadIdx.x % Tl;
-  const uint32_t row = blockIdx.x * uint32_t(A.cols) + line;
+  // Lines of 64 threads or more run one per block (the engine's launch), so
+  // the row is block-uniform.
+  constexpr bool ONE = Tl >= 64;
+  const uint32_t line = ONE ? 0u : threadIdx.x / Tl, tg = ONE ? threadIdx.x : threadIdx.x % Tl;
+  const uint32_t row = ONE ? blockIdx.x : blockIdx.x * uint32_t(A.cols) + line;
   const size_t base = so + size_t(k) * n + size_t(row) * S;
   const uint32_t lineRoot = (1u << A.m1) + row;
   const TwDev* twf = J.fwd + size_t(k) * n;
@@ -382,8 +389,9 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
 
   // Roots one round ahead, except for rows of 2^12 (three full radix-16
   // rounds), where the extra registers would leave three waves per SIMD
-  // instead of four.
-  constexpr bool AH = (M != 12);
+  // instead of four, and rows of 2^11, whose five waves per SIMD leave 96
+  // registers.
+  constexpr bool AH = (M != 12 && M != 11);
   uint32_t x[1 << LR];
   SchedTw<M, LR, 0> t0;
   if constexpr (AH) round_tw(t0, M, 0, tg, lineRoot, twf);
@@ -414,9 +422,30 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
         load_opv();
       }
     });
-    if constexpr (!AH) load_opv();
+    // Rows of 2^11 (96 registers): the operand is loaded one register group
+    // at a time, each group's loads kept after the previous group's products
+    // (the scheduling barrier), so that the 16 operand words are never live
+    // together.
+    constexpr bool GROUP_OP = (M == 11);
+    if constexpr (!AH && !GROUP_OP) load_opv();
     // Pointwise product on the leaves held in registers.
     const uint32_t q = A.q[k], sc = A.sc[k], scq = A.scq[k];
+    if constexpr (GROUP_OP && MODE != ROW_SQR) {
+      const uint32_t* op = J.opnd + base;
+#pragma unroll
+      for (int s = 0; s < ((1 << LR) >> RL); ++s) {
+        uint32_t node;
+        const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
+#pragma unroll
+        for (int j = 0; j < (1 << RL); ++j) {
+          uint32_t& v = x[s * (1 << RL) + j];
+          const uint32_t o = op[b + uint32_t(j)];
+          if constexpr (MODE == ROW_MULC) v = mont(v, o, p, q);
+          else v = mont_c(mont(v, o, p, q), sc, scq, p);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else
 #pragma unroll
     for (int i = 0; i < (1 << LR); ++i) {
       uint32_t& v = x[i];

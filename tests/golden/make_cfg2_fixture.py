@@ -7,13 +7,16 @@ The residues come from proth_res64.cpp (GMP, exact integer arithmetic: a^k
 by mpz_powm, then modular squarings with a special-form reduction checked
 against mpz_mod).  Stored: k, n, base a, is_prp, res64 (low 64 bits of
 a^(N-1) mod N, src/annie.h:355-366) and the low 64 bits of a^(k 2^j) mod N at
-a few j.  Run once in the build container; takes hours (six threads)."""
+a few j.  Run once in the build container; takes hours (six threads).
+Usage: make_cfg2_fixture.py [STATE_FILE] (resumable; default under the
+system temporary directory)."""
 import json
 import math
 import os
 import random
 import subprocess
 import sys
+import tempfile
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,7 +43,9 @@ def seeded_k():
 
 
 def main():
-    state = sys.argv[1] if len(sys.argv) > 1 else os.path.join(ROOT, "_work", "cfg2_state.bin")
+    # Scratch (the generator binary and its resumable state) outside the tree.
+    state = sys.argv[1] if len(sys.argv) > 1 else os.path.join(tempfile.gettempdir(), "annie_cfg2", "state.bin")
+    os.makedirs(os.path.dirname(state), exist_ok=True)
     k, a, plan = seeded_k()
     print(f"k = {k}, base {a}, plan {plan}", flush=True)
     exe = os.path.join(os.path.dirname(state), "proth_res64")

@@ -88,6 +88,13 @@ def _cfg4_tests(fx):
     return [{"N": proth(e["k"], e["n"]), "is_euler": True} for e in fx]
 
 
+def _batch_key(plan):
+    """The driver's batch of a plan: (K, pck, LN, digit-width class), plans
+    that differ only in lbase within a width class share a batch."""
+    K, pck, LN, lbase = plan
+    return (K, pck, LN, (pck * lbase) >> 5)
+
+
 def test_cfg4_sample_batched_matches_gmp(gpu, capsys):
     fx = _load("config_res64.json")["cfg4"]
     plans = {tuple(e["plan"]) for e in fx}
@@ -96,7 +103,7 @@ def test_cfg4_sample_batched_matches_gmp(gpu, capsys):
     _say(capsys, f"case cfg4 sample: {len(tests)} Proth tests in {len(plans)} length classes")
     res, st = annie_amd.batch_run(tests, engine="gpu")
     _say(capsys, f"  {st}")
-    assert st["rc"] == 0 and st["classes"] == len(plans)
+    assert st["rc"] == 0 and st["classes"] == len({_batch_key(p) for p in plans})
     for e, r in zip(fx, res):
         _expect(r, e, f"{e['k']}*2^{e['n']}+1")
 
@@ -127,14 +134,14 @@ def test_hbm_budget_forces_queued_admission(gpu, capsys):
     fx = _load("config_res64.json")["cfg4"]
     ents = [e for e in fx if e["plan"][2] <= 10][:9]
     tests = _cfg4_tests(ents)
-    classes = {tuple(e["plan"]) for e in ents}
+    classes = {_batch_key(e["plan"]) for e in ents}
     per_class = {}
     for e in ents:
-        per_class[tuple(e["plan"])] = per_class.get(tuple(e["plan"]), 0) + 1
+        per_class[_batch_key(e["plan"])] = per_class.get(_batch_key(e["plan"]), 0) + 1
     assert max(per_class.values()) >= 2
     # slots of a class = max(1, floor((budget / classes - fixed) / per_slot)):
     # below classes * (fixed + 2 per_slot) for every class, that is one slot
-    sizes = [annie_amd.batch_bytes(K, LN) for K, pck, LN, lbase in classes]
+    sizes = [annie_amd.batch_bytes(K, LN) for K, pck, LN, width in classes]
     budget_mb = len(classes) * min(fixed + 1.5 * slot for slot, fixed in sizes) / 2 ** 20
     res, st = annie_amd.batch_run(tests, engine="gpu", hbm_budget_mb=budget_mb)
     _say(capsys, f"case budget {budget_mb:.1f} MiB: {st}")
@@ -146,7 +153,7 @@ def test_hbm_budget_forces_queued_admission(gpu, capsys):
     order = {}
     for e, r in zip(ents, res):
         if r["status"] != "factor":
-            order.setdefault(tuple(e["plan"]), []).append(r)
+            order.setdefault(_batch_key(e["plan"]), []).append(r)
     for rs in order.values():
         rs = sorted(rs, key=lambda x: x["admitted"])
         for prev, nxt in zip(rs, rs[1:]):
