@@ -62,6 +62,33 @@ def test_cfg1_fermat_332000_bits(gpu, capsys):
         assert f"res64 = {fx['res64']:016X}" in ref["results"]
 
 
+def test_cfg2_proth_3M_prefix_matches_gmp_fixture(gpu, capsys):
+    """The plan class of k*2^3,000,000+1 (K=4, pck=2, LN=16, lbase=27) on the
+    test's own exponent bits: after bits(k) + j square_mul steps the register
+    holds a^(k 2^j) mod N, whose low 64 bits the fixture holds (GMP) for
+    j = 1,000, 20,000 and 100,000."""
+    fx = _load("cfg2_proth.json")
+    k, a = fx["k"], fx["a"]
+    N = proth(k, fx["n"])
+    pts = sorted((k.bit_length() + p["j"], p["low64"]) for p in fx["prefix"] if p["j"] <= 100_000)
+    e = (N - 1) >> 1
+    bits = [int(c) for c in format(e >> (e.bit_length() - pts[-1][0]), "b")]
+    T = annie_amd.Transform(N, a, min_ln=10)
+    try:
+        assert not T.init_failed() and (T.K, T.pck, T.LN, T.lbase) == tuple(fx["plan"])
+        T.set1()
+        done = 0
+        for it, low in pts:
+            T.square_mul_bits(bits[done:it], a)
+            done = it
+            got = T.get(0) & ((1 << 64) - 1)
+            _say(capsys, f"case cfg2 prefix: iteration {it}, low 64 bits {got:016X}")
+            assert got == low, (it, f"{got:016X}", f"{low:016X}")
+    finally:
+        T.close()
+
+
+@pytest.mark.long
 def test_cfg2_proth_3M_matches_gmp_fixture(gpu, capsys, tmp_path):
     """The Euler test of k*2^3,000,000+1, about 3 million squarings, run in
     chunks of 600,000 loop iterations: each chunk ends with a checkpoint in
