@@ -152,6 +152,19 @@ __device__ __forceinline__ void round_store(const uint32_t (&x)[1 << LR], int m,
   }
 }
 
+// Word i of a root table, line or operand (i < 2^30: tables and planes are at
+// most 2^23 words), addressed by a 32-bit byte offset from the base, so the
+// compiler can use the scalar-base + 32-bit vector-offset form instead of a
+// 64-bit address per thread (fewer VGPRs: a row pass of 2^9 drops from 85 to
+// 64 and runs eight waves per SIMD instead of five).
+__device__ __forceinline__ uint32_t ld_w(const uint32_t* base, uint32_t i) {
+  return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(base) + (i << 2));
+}
+
+__device__ __forceinline__ void st_w(uint32_t* base, uint32_t i, uint32_t v) {
+  *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + (i << 2)) = v;
+}
+
 // Load the roots of round (d0, RR) of the line whose root node is lineRoot
 // (heap index at level 0 of this pass).
 template <int LR, int RR>
@@ -165,7 +178,7 @@ __device__ __forceinline__ void round_tw(RoundTw<LR, RR>& T, int m, int d0, uint
 #pragma unroll
     for (int e = 0; e < RR; ++e)
 #pragma unroll
-      for (int blk = 0; blk < (1 << e); ++blk) T.w[s * RoundTw<LR, RR>::W + (1 << e) - 1 + blk] = tw[(H0 << e) + blk];
+      for (int blk = 0; blk < (1 << e); ++blk) T.w[s * RoundTw<LR, RR>::W + (1 << e) - 1 + blk] = ld_w(tw, (H0 << e) + blk);
   }
 }
 
@@ -299,8 +312,8 @@ __device__ __forceinline__ void row_line(const NttJob& J, const NttArgs& A, int 
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
 
-  auto gld = [&](uint32_t i) { return src[i]; };
-  auto gst = [&](uint32_t i, uint32_t v) { dst[i] = v; };
+  auto gld = [&](uint32_t i) { return ld_w(src, i); };
+  auto gst = [&](uint32_t i, uint32_t v) { st_w(dst, i, v); };
   auto lld = [&](uint32_t i) { return X[lpad(i)]; };
   auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
 
@@ -325,7 +338,7 @@ __device__ __forceinline__ void row_line(const NttJob& J, const NttArgs& A, int 
           uint32_t node;
           const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
 #pragma unroll
-          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
+          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = ld_w(op, b + uint32_t(j));
         }
       }
     };
@@ -382,8 +395,8 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
   const uint32_t* src = J.src + base;
   uint32_t* dst = J.dst + base;
 
-  auto gld = [&](uint32_t i) { return src[i]; };
-  auto gst = [&](uint32_t i, uint32_t v) { dst[i] = v; };
+  auto gld = [&](uint32_t i) { return ld_w(src, i); };
+  auto gst = [&](uint32_t i, uint32_t v) { st_w(dst, i, v); };
   auto lld = [&](uint32_t i) { return X[lpad(i)]; };
   auto lst = [&](uint32_t i, uint32_t v) { X[lpad(i)] = v; };
 
@@ -412,7 +425,7 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
           uint32_t node;
           const uint32_t b = group_base<LR, RL>(M, DL, tg, s, node);
 #pragma unroll
-          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = op[b + uint32_t(j)];
+          for (int j = 0; j < (1 << RL); ++j) opv[s * (1 << RL) + j] = ld_w(op, b + uint32_t(j));
         }
       }
     };
@@ -439,7 +452,7 @@ __global__ void __launch_bounds__(RowGeo<M>::THREADS, RowGeo<M>::WAVES_PER_EU) k
 #pragma unroll
         for (int j = 0; j < (1 << RL); ++j) {
           uint32_t& v = x[s * (1 << RL) + j];
-          const uint32_t o = op[b + uint32_t(j)];
+          const uint32_t o = ld_w(op, b + uint32_t(j));
           if constexpr (MODE == ROW_MULC) v = mont(v, o, p, q);
           else v = mont_c(mont(v, o, p, q), sc, scq, p);
         }
