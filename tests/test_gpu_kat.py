@@ -40,9 +40,9 @@ def test_quick_test_list_batched(gpu):
 # The rest of the QUICK_TEST list (src/main.cpp:163) above 400,000 bits, run
 # through the command-line front end (the expression evaluated by the CLI's
 # parser, the test on the GPU engine, src/main.cpp:246-264): each is a
-# Fermat probable prime with every Gerbicz-Li check passing (the largest,
-# about 2.5 minutes, with the long tests).
-CLI_QUICK = [590041, 1285607, pytest.param(2904353, marks=pytest.mark.long)]
+# Fermat probable prime with every Gerbicz-Li check passing (the two largest,
+# about 1 and 2.5 minutes, with the long tests).
+CLI_QUICK = [590041, pytest.param(1285607, marks=pytest.mark.long), pytest.param(2904353, marks=pytest.mark.long)]
 
 
 @pytest.mark.parametrize("i", CLI_QUICK)
