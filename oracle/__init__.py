@@ -63,6 +63,8 @@ def lib():
         L.or_gmp_raw.restype = ctypes.c_long
         L.or_powm.argtypes = [u64p, sz, u64p, sz, u64p, sz, u64p, sz]
         L.or_powm.restype = ctypes.c_long
+        L.or_gcd.argtypes = [u64p, sz, u64p, sz, u64p, sz]
+        L.or_gcd.restype = ctypes.c_long
         L.or_base.argtypes = [u64p, sz, ctypes.c_int, ctypes.c_int32]
         L.or_base.restype = ctypes.c_int32
         L.or_check.argtypes = [u64p, sz, ctypes.c_int, ctypes.c_int32, ip, u64p, ip, ctypes.c_long, u64p, sz,
@@ -123,6 +125,17 @@ def powm(b, e, N):
     cap = nn + 1
     out = (ctypes.c_uint64 * cap)()
     r = L.or_powm(B, nb, E, ne, M, nn, out, cap)
+    return from_limbs(out, r)
+
+
+def gcd(a, b):
+    """gcd(a, b) by GMP (subquadratic; Python's math.gcd is quadratic)."""
+    L = lib()
+    A, na = limbs(a)
+    B, nb = limbs(b)
+    cap = min(na, nb) + 1
+    out = (ctypes.c_uint64 * cap)()
+    r = L.or_gcd(A, na, B, nb, out, cap)
     return from_limbs(out, r)
 
 

@@ -170,6 +170,19 @@ long or_powm(const uint64_t* b, size_t nb, const uint64_t* e, size_t ne, const u
   return long(r);
 }
 
+// gcd(a, b) with GMP's subquadratic algorithm (Python's is quadratic: tens
+// of seconds at millions of bits); pinned against math.gcd on small inputs.
+long or_gcd(const uint64_t* a, size_t na, const uint64_t* b, size_t nb, uint64_t* out, size_t cap) {
+  mpz_t A, B, z;
+  mpz_inits(A, B, z, nullptr);
+  from_limbs(A, a, na);
+  from_limbs(B, b, nb);
+  mpz_gcd(z, A, B);
+  const size_t r = to_limbs(z, out, cap);
+  mpz_clears(A, B, z, nullptr);
+  return long(r);
+}
+
 // Base of the test (src/annie.h:249-254): 2 for Fermat unless a_ref != 0; for
 // Euler the smallest a >= 2, skipping squares, with Jacobi (a/N) = -1.
 int32_t or_base(const uint64_t* Nl, size_t nl, int is_euler, int32_t a_ref) {

@@ -20,7 +20,7 @@ def usable_N(bits, seed, a=3, min_ln=4):
         if LN < min_ln:
             raise ValueError("size below the engine's minimum length")
         R = (1 << (pck * lbase << LN)) - 1
-        if math.gcd(N, R) == 1:
+        if oracle.gcd(N, R) == 1:
             return N
 
 
