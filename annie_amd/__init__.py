@@ -77,6 +77,9 @@ def load_library(path=HIP_LIB_PATH):
     lib.annie_hip_save_checkpoint.argtypes = [h, ctypes.c_void_p, sz]
     lib.annie_hip_load_checkpoint.argtypes = [h, ctypes.c_void_p, sz]
     lib.annie_hip_info.argtypes = [h, ip, ip, ip, ip]
+    lib.annie_hip_schedule_fits.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.annie_hip_set_schedule.argtypes = [h, ctypes.c_int, ctypes.c_int]
+    lib.annie_hip_get_schedule.argtypes = [h, ip, ip]
     lib.annie_hip_sync.argtypes = [h]
     lib.annie_hip_read_register.argtypes = [h, sz, ctypes.POINTER(ctypes.c_uint32), sz]
     lib.annie_hip_write_register.argtypes = [h, sz, ctypes.POINTER(ctypes.c_uint32), sz]
@@ -133,6 +136,44 @@ def plan_constants(LN, K, pck, lbase):
     v = list(out)
     return {"p": v[0:5], "q": v[5:10], "sq_scale": v[10:15], "c32": v[15:20], "c64": v[20:25],
             "ginv": [v[25 + 5 * i:30 + 5 * i] for i in range(5)]}
+
+
+def schedule_fits(LN, row_lines, col_cols):
+    """Whether an NTT launch schedule fits transforms of length 2^LN."""
+    return bool(load_library().annie_hip_schedule_fits(int(LN), int(row_lines), int(col_cols)))
+
+
+def schedule_default(LN):
+    """The engine's default launch schedule (row lines, column count) for
+    length 2^LN, 0 where a pass has no knob (rows of 64 threads or more,
+    columns of 2^10 or more, or no column pass)."""
+    m1 = 0 if LN <= 13 else LN // 2
+    m2 = LN - m1
+    tr = (1 << m2) // 16
+    lines = 0 if tr >= 64 else min(64 // tr, 1 << m1)
+    cols = 0
+    if 0 < m1 < 10:
+        cols = min(256 // ((1 << m1) // 16), 16, 1 << (LN - m1))
+    return lines, cols
+
+
+def schedules(LN):
+    """The launch schedules (row lines, column count) worth timing at length
+    2^LN: (0, 0) (the default) first, then every other pair that fits with
+    at least one 64-lane wave per workgroup in each pass that has a knob."""
+    m1 = 0 if LN <= 13 else LN // 2
+    m2 = LN - m1
+    tr, tc = (1 << m2) // 16, (1 << m1) // 16
+    dl, dc = schedule_default(LN)
+    pw = [1 << i for i in range(11)]
+    ls = [x for x in pw if x * tr >= 64] if dl else [0]
+    cs = [x for x in pw if x * tc >= 64] if dc else [0]
+    out = [(0, 0)]
+    for rl in ls:
+        for cc in cs:
+            if (rl, cc) != (dl, dc) and schedule_fits(LN, rl, cc):
+                out.append((rl, cc))
+    return out
 
 
 def batch_bytes(K, LN):
@@ -253,6 +294,18 @@ class Transform:
     def has_checkpoint(self):
         return True
 
+    def set_schedule(self, row_lines=0, col_cols=0):
+        """NTT launch schedule (lines per row-pass workgroup for short rows,
+        columns per column-pass workgroup for columns of up to 2^9); 0 keeps
+        the default.  Results do not depend on it."""
+        self._check(self._lib.annie_hip_set_schedule(self._h, int(row_lines), int(col_cols)))
+
+    def get_schedule(self):
+        """(row lines, column count) in effect; 0 where a pass has no knob."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.annie_hip_get_schedule(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def sync(self):
         self._check(self._lib.annie_hip_sync(self._h))
 
@@ -367,7 +420,8 @@ class BatchOpts(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int), ("device", ctypes.c_int), ("max_slots", ctypes.c_size_t),
                 ("hbm_budget_mb", ctypes.c_double), ("realtime", ctypes.c_int), ("time_limit", ctypes.c_double),
                 ("verbose", ctypes.c_int), ("graphs", ctypes.c_int), ("flush_ticks", ctypes.c_int),
-                ("run_steps", ctypes.c_int), ("run_min_active", ctypes.c_int), ("host_threads", ctypes.c_int)]
+                ("run_steps", ctypes.c_int), ("run_min_active", ctypes.c_int), ("host_threads", ctypes.c_int),
+                ("row_lines", ctypes.c_int), ("col_cols", ctypes.c_int)]
 
 
 class BatchResult(ctypes.Structure):
@@ -383,7 +437,8 @@ class BatchStats(ctypes.Structure):
 
 
 def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, realtime=False, time_limit=0.0,
-              verbose=False, graphs=False, flush_ticks=0, run_steps=0, run_min_active=0, host_threads=0):
+              verbose=False, graphs=False, flush_ticks=0, run_steps=0, run_min_active=0, host_threads=0,
+              row_lines=0, col_cols=0):
     """Run many tests together, one engine batch per length class, with
     continuous admission (annie_batch_run in include/annie_host.h).
 
@@ -394,6 +449,8 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
     to run_steps squarings per test in one launch (squaring runs, lengths
     2^9 .. 2^13); results are the same either way.  host_threads: host
     workers driving the classes (0 = one per class, at most 16).
+    row_lines / col_cols: NTT launch schedule (Transform.set_schedule) of
+    every class whose transform length it fits; 0 = default.
     Returns (results, stats): one dict per test (status, is_prp, res64, gl_ok,
     a, iterations, arrival, admitted, finished, message) and a stats dict.
     """
@@ -411,7 +468,7 @@ def batch_run(tests, engine="gpu", device=0, max_slots=0, hbm_budget_mb=0.0, rea
         arr[i] = BatchTest(limbs, n, 1 if t.get("is_euler") else 0, int(t.get("a", 0)), float(t.get("arrival", 0.0)))
     opts = BatchOpts(0 if engine == "gpu" else 1, device, max_slots, hbm_budget_mb, 1 if realtime else 0,
                      time_limit, 1 if verbose else 0, 1 if graphs else 0, int(flush_ticks), int(run_steps),
-                     int(run_min_active), int(host_threads))
+                     int(run_min_active), int(host_threads), int(row_lines), int(col_cols))
     res = (BatchResult * max(len(tests), 1))()
     st = BatchStats()
     rc = lib.annie_batch_run(arr, len(tests), ctypes.byref(opts), res, ctypes.byref(st))

@@ -92,6 +92,10 @@ struct Ctx {
   Plan pl;
   int device = 0;
   int m1 = 0, m2 = 0;
+  // Launch schedule of the NTT passes (0: the default): lines per row-pass
+  // workgroup for rows of fewer than 64 threads, columns per column-pass
+  // workgroup for columns of up to 2^9 (annie_hip_set_schedule).
+  uint32_t sched_lines = 0, sched_cols = 0;
   size_t n = 0;
   hipStream_t stream = nullptr;
   // Stream of the next kernel launch: `stream`, or `aux` while the
@@ -220,6 +224,50 @@ static void launch_col_m(bool inverse, dim3 g, dim3 b, size_t shm, hipStream_t s
   else hipLaunchKernelGGL((k_col_fwd<M>), g, b, shm, s, A);
 }
 
+// Default launch geometry of the passes.  Row pass: one 64-lane wave per
+// workgroup for short rows (64 / Tl lines of Tl threads), one line per
+// workgroup otherwise.  Column pass: columns of 2^10 and 2^11 take 8 columns
+// per workgroup (a compile-time constant of their kernels); shorter columns
+// at most 256 threads and 16 columns per workgroup, and at most the number
+// of columns (columns of 2^9: 8 rather than 16, measured 3% faster per
+// squaring at 2^19 by the schedule autotuner and even at 2^18).
+static uint32_t row_lines_default(int m1, int m2) {
+  const uint32_t Tl = (1u << m2) / NTT_R, rows = 1u << m1;
+  uint32_t lines = (Tl >= 64) ? 1u : 64u / Tl;
+  return lines > rows ? rows : lines;
+}
+
+static uint32_t col_cols_default(int m1, int LN) {
+  if (m1 >= 10) return 8u;
+  const uint32_t Tl = (1u << m1) / NTT_R, n2 = 1u << (LN - m1);
+  uint32_t C = 256u / Tl;
+  if (C > 16) C = 16;
+  if (C > n2) C = n2;
+  return C;
+}
+
+// Whether a schedule fits the passes of a plan with column levels m1 and row
+// levels m2 (0 = default for either).  Row lines: a power of two, at most the
+// number of rows, at most 512 threads and 64 KiB of LDS per workgroup, rows
+// of fewer than 64 threads only.  Column count: a power of two, at most the
+// number of columns, at most 1024 threads and 64 KiB of LDS (data and staged
+// roots) per workgroup, columns of up to 2^9 only.
+static bool schedule_fits(int m1, int m2, int LN, uint32_t lines, uint32_t cols) {
+  auto pow2 = [](uint32_t v) { return v != 0 && (v & (v - 1)) == 0; };
+  if (lines != 0) {
+    const uint32_t S = 1u << m2, Tl = S / NTT_R, rows = 1u << m1;
+    if (!pow2(lines) || Tl >= 64 || lines > rows || lines * Tl > 512u) return false;
+    if (size_t(lines) * (S + (S >> 5)) * 4 > 65536) return false;
+  }
+  if (cols != 0) {
+    if (m1 == 0 || m1 >= 10) return false;
+    const uint32_t S = 1u << m1, Tl = S / NTT_R, n2 = 1u << (LN - m1);
+    if (!pow2(cols) || cols > n2 || cols * Tl > 1024u) return false;
+    if (size_t(cols + 1) * S * 4 > 65536) return false;
+  }
+  return true;
+}
+
 static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   NttArgs A = c.nbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
@@ -230,8 +278,8 @@ static int launch_row(Ctx& c, int mode, const NttJob* jobs, int nj) {
   A.m1 = c.m1;
   const uint32_t S = 1u << c.m2, Tl = S / NTT_R;
   const uint32_t rows = 1u << c.m1;
-  uint32_t lines = (Tl >= 64) ? 1u : 64u / Tl;
-  if (lines > rows) lines = rows;
+  uint32_t lines = row_lines_default(c.m1, c.m2);
+  if (c.sched_lines != 0 && Tl < 64) lines = c.sched_lines;  // checked by annie_hip_set_schedule
   A.cols = int(lines);
   const dim3 grid(rows / lines, c.pl.K, nj * c.nslots), block(lines * Tl);
   const size_t shm = size_t(lines) * (S + (S >> 5)) * sizeof(uint32_t);
@@ -267,10 +315,8 @@ static int launch_col(Ctx& c, bool inverse, const NttJob* jobs, int nj) {
   const bool wide = c.m1 >= 10;
   const uint32_t S = 1u << c.m1, Tl = S / (wide ? 32u : uint32_t(NTT_R));
   const uint32_t n2 = uint32_t(c.n >> c.m1);
-  uint32_t C = wide ? 8u : 1024u / Tl;
-  if (C < 1) C = 1;
-  if (C > 16) C = 16;
-  if (C > n2) C = n2;
+  uint32_t C = col_cols_default(c.m1, c.pl.LN);
+  if (c.sched_cols != 0 && !wide) C = c.sched_cols;  // checked by annie_hip_set_schedule
   // The 32-element column kernels take 8 columns per block as a compile-time
   // constant (n2 >= 2^10 whenever m1 >= 10, so C is never clipped there).
   if (wide && C != 8u) return fail(ANNIE_ERR_SIZE, "column pass geometry");
@@ -1074,6 +1120,42 @@ int annie_hip_info(const annie_hip_t* h, int* K, int* pck, int* LN, int* lbase) 
   if (pck) *pck = h->c.pl.pck;
   if (LN) *LN = h->c.pl.LN;
   if (lbase) *lbase = h->c.pl.lbase;
+  return ANNIE_OK;
+}
+
+static void plan_levels(int LN, int* m1, int* m2) {
+  *m1 = (LN <= 13) ? 0 : LN / 2;
+  *m2 = LN - *m1;
+}
+
+int annie_hip_schedule_fits(int LN, int row_lines, int col_cols) {
+  if (LN < 4 || LN > 23 || row_lines < 0 || col_cols < 0) return 0;
+  int m1, m2;
+  plan_levels(LN, &m1, &m2);
+  return schedule_fits(m1, m2, LN, uint32_t(row_lines), uint32_t(col_cols)) ? 1 : 0;
+}
+
+int annie_hip_set_schedule(annie_hip_t* h, int row_lines, int col_cols) {
+  if (!ready(h)) return ANNIE_ERR_ARG;
+  Ctx& c = h->c;
+  if (row_lines < 0 || col_cols < 0 || !schedule_fits(c.m1, c.m2, c.pl.LN, uint32_t(row_lines), uint32_t(col_cols)))
+    return fail(ANNIE_ERR_ARG, "schedule does not fit this transform length");
+  HIPCHK(hipStreamSynchronize(c.stream));
+  if (c.aux) HIPCHK(hipStreamSynchronize(c.aux));
+  for (auto& g : c.graphs) (void)hipGraphExecDestroy(g.exec);
+  c.graphs.clear();
+  c.sched_lines = uint32_t(row_lines);
+  c.sched_cols = uint32_t(col_cols);
+  return ANNIE_OK;
+}
+
+int annie_hip_get_schedule(const annie_hip_t* h, int* row_lines, int* col_cols) {
+  if (!h) return fail(ANNIE_ERR_ARG, "null context");
+  const Ctx& c = h->c;
+  const bool short_rows = (1u << c.m2) / NTT_R < 64;
+  const bool narrow_cols = c.m1 > 0 && c.m1 < 10;
+  if (row_lines) *row_lines = short_rows ? int(c.sched_lines ? c.sched_lines : row_lines_default(c.m1, c.m2)) : 0;
+  if (col_cols) *col_cols = narrow_cols ? int(c.sched_cols ? c.sched_cols : col_cols_default(c.m1, c.pl.LN)) : 0;
   return ANNIE_OK;
 }
 

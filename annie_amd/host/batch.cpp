@@ -83,10 +83,12 @@ class HipBatch : public BatchEngine {
 
  public:
   // lbase: that of one of the class's moduli.
-  HipBatch(const PlanKey& k, int lbase, size_t slots, int device, bool graphs) {
+  HipBatch(const PlanKey& k, int lbase, size_t slots, int device, bool graphs, int row_lines, int col_cols) {
     const int K = std::get<0>(k), pck = std::get<1>(k), LN = std::get<2>(k);
     check(annie_hip_batch_create(K, pck, LN, lbase, slots, device, &h_), "batch create");
     if (graphs) check(annie_hip_batch_set_graphs(h_, 1), "graphs");
+    if ((row_lines != 0 || col_cols != 0) && annie_hip_schedule_fits(LN, row_lines, col_cols))
+      check(annie_hip_set_schedule(h_, row_lines, col_cols), "schedule");
     // limbs of a value below B^n with the widest digit of the class
     stride_ = ((size_t(32) * size_t(std::get<3>(k) + 1)) << LN) / 64 + 2;
   }
@@ -525,7 +527,7 @@ static int batch_run(const annie_batch_test* tests, size_t count, const annie_ba
           const double ts = now();
           if (!c.eng) {
             if (o.engine == 1) c.eng.reset(new GmpBatch(c.slots));
-            else c.eng.reset(new HipBatch(c.key, c.lbase, c.slots, o.device, o.graphs != 0));
+            else c.eng.reset(new HipBatch(c.key, c.lbase, c.slots, o.device, o.graphs != 0, o.row_lines, o.col_cols));
             c.slot.assign(c.slots, nullptr);
           }
           size_t s = 0;

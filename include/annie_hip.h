@@ -115,6 +115,22 @@ int annie_hip_load_checkpoint(annie_hip_t* ctx, const void* buf, size_t size);
 /* get_info_short fields (src/transformGPU.h:784-789). */
 int annie_hip_info(const annie_hip_t* ctx, int* K, int* pck, int* LN, int* lbase);
 
+/* Launch schedule of the NTT passes (no counterpart in the reference, whose
+ * OpenCL kernels have fixed work-group shapes, src/engine.h:262-339): the
+ * number of lines per row-pass workgroup for rows of fewer than 64 threads
+ * (rows of up to 2^9 elements at 16 per thread) and the number of columns per
+ * column-pass workgroup for columns of up to 2^9 elements; 0 keeps the
+ * default.  Results do not depend on the schedule.  Values must be powers of
+ * two that fit the plan (at most the rows / columns there are, 512 row-pass
+ * or 1024 column-pass threads, 64 KiB of LDS per workgroup):
+ * annie_hip_schedule_fits checks a pair for a transform length without a
+ * context; set_schedule returns ANNIE_ERR_ARG for a pair that does not fit
+ * and drops the context's captured graphs.  get_schedule reports the values
+ * in effect (the defaults where 0 was set; 0 where a pass has no knob). */
+int annie_hip_schedule_fits(int LN, int row_lines, int col_cols);
+int annie_hip_set_schedule(annie_hip_t* ctx, int row_lines, int col_cols);
+int annie_hip_get_schedule(const annie_hip_t* ctx, int* row_lines, int* col_cols);
+
 /* Block until all queued work of the context is done. */
 int annie_hip_sync(annie_hip_t* ctx);
 

@@ -83,6 +83,8 @@ typedef struct {
   int run_min_active;         /* running tests a class needs for squaring runs, 0 = 1 */
   int host_threads;           /* host worker threads driving the classes (class i to worker i mod W),
                                  0 = min(classes, 16); one when graphs is set */
+  int row_lines, col_cols;    /* NTT launch schedule of every class whose transform length it fits
+                                 (annie_hip_set_schedule in annie_hip.h); 0 = default */
 } annie_batch_opts;
 
 typedef struct {

@@ -154,3 +154,25 @@ def test_bench_reads_committed_traffic_for_every_profiled_kernel():
         assert c.get("traffic") is not None, kind
         assert c["traffic"] >= 0.95 * bench.alg_bytes(kind, U), kind
         assert c.get("valu_util") is not None and 0.0 < c["valu_util"] <= 1.0, kind
+
+
+def test_schedule_fits_and_defaults():
+    """The launch-schedule rules of annie_hip_schedule_fits (no GPU needed):
+    powers of two only; row lines only for rows of fewer than 64 threads, at
+    most the rows there are and 512 threads; columns only for column passes
+    of up to 2^9, at most 1024 threads and 64 KiB of LDS; the defaults fit."""
+    fits = annie_amd.schedule_fits
+    assert fits(16, 4, 16) and fits(16, 32, 32) and fits(14, 64, 64)
+    assert not fits(16, 3, 0) and not fits(16, 0, 24)      # not powers of two
+    assert not fits(16, 64, 0)                             # 64 lines x 16 threads > 512
+    assert not fits(14, 0, 128)                            # (128 + 1) x 2^7 words > 64 KiB
+    assert not fits(13, 2, 0) and not fits(13, 0, 8)       # one-pass 2^13: no knobs
+    assert not fits(21, 0, 8) and not fits(21, 1, 0)       # 2^10 columns, rows of 2^11
+    assert fits(9, 1, 0) and not fits(9, 2, 0)             # one-pass 2^9: one row
+    assert not fits(16, -1, 0)
+    for LN in range(9, 23):
+        d = annie_amd.schedule_default(LN)
+        assert fits(LN, *d), LN
+        sc = annie_amd.schedules(LN)
+        assert sc[0] == (0, 0) and len(set(sc)) == len(sc) and d not in sc[1:]
+        assert all(fits(LN, *x) for x in sc)
