@@ -56,4 +56,3 @@ def test_integration_binding_compiles_and_links(tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
-
