@@ -119,7 +119,19 @@ struct Ctx {
   // Bits per coefficient of each slot (batches: moduli of one (K, LN) whose
   // plans differ in lbase share a batch), and their device copy.
   std::vector<int> slot_cb;
+  // One device allocation of 3 words per slot: digit widths (cbits_dev),
+  // bases (base_dev: the test's a, for multiplier bits) and the slot list of
+  // the last batched squaring with short-list bits (list_dev, whose host copy
+  // is list_host; re-uploaded only when the list changes).
   uint32_t* cbits_dev = nullptr;
+  uint32_t* base_dev = nullptr;
+  uint32_t* list_dev = nullptr;
+  std::vector<uint32_t> slot_base, list_host;
+  // Per-position bits of the batched squaring being issued (bits_count
+  // positions, 0 = none): multipliers for its first carry, snapshot flags for
+  // its last.
+  size_t bits_count = 0;
+  uint32_t mulbits[INLINE_SLOTS / 32] = {}, snapbits[INLINE_SLOTS / 32] = {};
   // Batched squaring replayed from captured HIP graphs: the slot list and
   // multipliers live at fixed device addresses (graph_buf: ids, then
   // multipliers), one executable graph per list length (grid sizes).
@@ -367,8 +379,10 @@ static int convolve(Ctx& c, int mode, const NttJob* jobs, int nj) {
   return launch_row(c, mode, jobs, nj);
 }
 
+// bits (batched squarings with short-list bits, c.bits_count > 0): 1 = this
+// carry takes the per-position multipliers, 2 = the snapshot flags.
 static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_slot = nullptr,
-                        const uint32_t* snap = nullptr) {
+                        const uint32_t* snap = nullptr, int bits = 0) {
   CarryArgs A = c.cbase;
   for (int i = 0; i < nj; ++i) A.job[i] = jobs[i];
   A.slot_ids = c.slot_ids;
@@ -376,6 +390,11 @@ static int launch_carry(Ctx& c, const CarryJob* jobs, int nj, const uint32_t* a_
   A.njobs = nj;
   A.a_slot = a_slot;
   A.snap = snap;
+  if (bits != 0 && c.bits_count > 0) {
+    A.bits_mode = bits;
+    A.slot_base = c.base_dev;
+    std::memcpy(A.bits, bits == 1 ? c.mulbits : c.snapbits, sizeof(A.bits));
+  }
   const size_t nb = c.n / CARRY_L;
   // CARRY_WG - 1 owned blocks per workgroup (thread 0 only hands over a carry).
   const dim3 grid(unsigned((nb + CARRY_WG - 2) / (CARRY_WG - 1)), nj * c.nslots), block(CARRY_WG);
@@ -438,7 +457,7 @@ static SqBufs ctx_bufs(const Ctx& c) {
 // the last carry.  The (m' + 2) N product works in B.sq, so B.wq may be in
 // use on aux.
 static int reduce(Ctx& c, const SqBufs& B, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr,
-                  bool join = false) {
+                  bool join = false, bool bits = false) {
   int rc;
   NttJob j = job(B.tr, B.wr, c.fwdR(), c.invR(), c.reg(RNIR));
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
@@ -448,8 +467,8 @@ static int reduce(Ctx& c, const SqBufs& B, uint32_t* dst, bool raw_out, const ui
   if ((rc = convolve(c, ROW_MULC, &j, 1))) return rc;
   if (join) HIPCHK(hipStreamWaitEvent(c.stream, c.ev_join, 0));
   cj = cjob(B.sq, dst, 1u, 1, 0, raw_out ? 1 : 0, B.tq);
-  if (snap != nullptr) cj.dst2 = c.reg(RSNAP);
-  return launch_carry(c, &cj, 1, nullptr, snap);
+  if (snap != nullptr || bits) cj.dst2 = c.reg(RSNAP);
+  return launch_carry(c, &cj, 1, nullptr, snap, bits ? 2 : 0);
 }
 
 static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nullptr, bool join = false) {
@@ -459,9 +478,9 @@ static int reduce(Ctx& c, uint32_t* dst, bool raw_out, const uint32_t* snap = nu
 // squareMMGW (src/transformGPU.h:556-568): B.y = a B.x^2.  a_slot: per-slot
 // multipliers of a batch operation (device array in slot-list order),
 // replacing a.
-static int square_mul(Ctx& c, const SqBufs& B, uint32_t a, const uint32_t* a_slot = nullptr) {
+static int square_mul(Ctx& c, const SqBufs& B, uint32_t a, const uint32_t* a_slot = nullptr, bool bits = false) {
   int rc;
-  if (c.aux != nullptr && a_slot == nullptr && !c.profiling) {
+  if (c.aux != nullptr && a_slot == nullptr && !bits && !c.profiling) {
     // Two streams (not while per-kernel timing is on, so that each kernel is
     // timed alone): the negacyclic half (x^2 mod Q -> T_Q) on aux runs beside
     // the m' chain on the main stream, whose one-job passes leave much of the
@@ -488,12 +507,12 @@ static int square_mul(Ctx& c, const SqBufs& B, uint32_t a, const uint32_t* a_slo
   const NttJob j[2] = {job(B.x, B.wr, c.fwdR(), c.invR()), job(B.x, B.wq, c.fwdQ(), c.invQ())};
   if ((rc = convolve(c, ROW_SQR, j, 2))) return rc;
   const CarryJob cj[2] = {cjob(B.wr, B.tr, a, 0, 0, 0), cjob(B.wq, B.tq, a, 1, 0, 1)};
-  if ((rc = launch_carry(c, cj, 2, a_slot))) return rc;
-  return reduce(c, B, B.y, false, a_slot);
+  if ((rc = launch_carry(c, cj, 2, a_slot, nullptr, bits ? 1 : 0))) return rc;
+  return reduce(c, B, B.y, false, a_slot, false, bits);
 }
 
-static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr) {
-  return square_mul(c, ctx_bufs(c), a, a_slot);
+static int square_mul(Ctx& c, uint32_t a, const uint32_t* a_slot = nullptr, bool bits = false) {
+  return square_mul(c, ctx_bufs(c), a, a_slot, bits);
 }
 
 // mulMMGW (src/transformGPU.h:540-554).
@@ -1298,10 +1317,14 @@ int annie_hip_batch_create(int K, int pck, int LN, int lbase, size_t slots, int 
   if ((rc = setup_device(c, device, slots))) return rc;
   c.slotN.resize(slots);
   c.slot_cb.assign(slots, c.pl.cbits());
-  if (hipMalloc(&c.cbits_dev, slots * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "per-slot digit widths");
+  c.slot_base.assign(slots, 1u);
+  if (hipMalloc(&c.cbits_dev, 3 * slots * 4) != hipSuccess) return fail(ANNIE_ERR_NOMEM, "per-slot digit widths");
+  c.base_dev = c.cbits_dev + slots;
+  c.list_dev = c.cbits_dev + 2 * slots;
   {
-    std::vector<uint32_t> cb(slots, uint32_t(c.pl.cbits()));
-    HIPCHK(hipMemcpy(c.cbits_dev, cb.data(), slots * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> cb(3 * slots, 1u);
+    std::fill(cb.begin(), cb.begin() + slots, uint32_t(c.pl.cbits()));
+    HIPCHK(hipMemcpy(c.cbits_dev, cb.data(), 3 * slots * 4, hipMemcpyHostToDevice));
   }
   c.cbase.cbits_slot = c.cbits_dev;
   Stage& st = c.stage;
@@ -1317,8 +1340,10 @@ int annie_hip_batch_bytes(int K, int LN, size_t* per_slot, size_t* fixed) {
   if (K < 2 || K > KMAX || LN < 4 || LN > 23) return fail(ANNIE_ERR_ARG, "bad plan");
   const size_t n = size_t(1) << LN;
   // setup_device's slot stride, the staging ring's two words per slot per
-  // entry and the graph buffers' two words per slot.
-  if (per_slot) *per_slot = (size_t(NREG) * size_t(K) * n + 2 * size_t(DIGIT_LIMBS) * n) * 4 + NSTAGE * 2 * 4 + 2 * 4;
+  // entry, the graph buffers' two words per slot and the per-slot width,
+  // base and list words.
+  if (per_slot)
+    *per_slot = (size_t(NREG) * size_t(K) * n + 2 * size_t(DIGIT_LIMBS) * n) * 4 + NSTAGE * 2 * 4 + 2 * 4 + 3 * 4;
   if (fixed) *fixed = 4 * size_t(K) * n * sizeof(TwDev);
   return ANNIE_OK;
 }
@@ -1348,8 +1373,10 @@ int annie_hip_batch_load(annie_hip_t* h, size_t slot, const uint64_t* N, size_t 
   }
   HIPCHK(hipSetDevice(c.device));
   c.slot_cb[slot] = pl.cbits();
-  const uint32_t cb = uint32_t(pl.cbits());
+  c.slot_base[slot] = uint32_t(a);
+  const uint32_t cb = uint32_t(pl.cbits()), ab = uint32_t(a);
   HIPCHK(hipMemcpyAsync(c.cbits_dev + slot, &cb, 4, hipMemcpyHostToDevice, c.stream));
+  HIPCHK(hipMemcpyAsync(c.base_dev + slot, &ab, 4, hipMemcpyHostToDevice, c.stream));
   HIPCHK(hipStreamSynchronize(c.stream));
   const int rc = load_slot_consts(c, slot, m->v, t);
   mpz_clears(R, t, nullptr);
@@ -1430,6 +1457,32 @@ int annie_hip_batch_square_mul(annie_hip_t* h, const uint32_t* slots, const uint
     if ((a[i] & ~SLOT_SNAP) < 1) return fail(ANNIE_ERR_ARG, "multiplier must be in [1, 2^31)");
   const uint32_t *d, *da;
   if (!c.use_graphs) {
+    // Short lists whose multipliers are 1 or the slot's base: the multipliers
+    // and snapshot flags travel as bits in the carries' kernel arguments and
+    // the slot list stays on the device until it changes, so a squaring in
+    // the steady state issues no host-to-device copy.
+    bool short_list = count <= size_t(INLINE_SLOTS);
+    for (size_t i = 0; i < count && short_list; ++i) {
+      const uint32_t m = a[i] & ~SLOT_SNAP;
+      short_list = m == 1u || m == c.slot_base[slots[i]];
+    }
+    if (short_list) {
+      if (c.list_host.size() != count || !std::equal(slots, slots + count, c.list_host.begin())) {
+        if ((rc = stage_upload(c, slots, nullptr, count, &d, nullptr, c.list_dev))) return rc;
+        c.list_host.assign(slots, slots + count);
+      }
+      std::memset(c.mulbits, 0, sizeof(c.mulbits));
+      std::memset(c.snapbits, 0, sizeof(c.snapbits));
+      for (size_t i = 0; i < count; ++i) {
+        if ((a[i] & ~SLOT_SNAP) != 1u) c.mulbits[i >> 5] |= 1u << (i & 31);
+        if (a[i] & SLOT_SNAP) c.snapbits[i >> 5] |= 1u << (i & 31);
+      }
+      c.bits_count = count;
+      SlotScope sc(c, c.list_dev, count);
+      rc = annie::square_mul(c, 1u, nullptr, true);
+      c.bits_count = 0;
+      return rc;
+    }
     if ((rc = stage_upload(c, slots, a, count, &d, &da))) return rc;
     SlotScope sc(c, d, count);
     return annie::square_mul(c, 1u, da);

@@ -64,6 +64,11 @@ struct CarryJob {
 // Bit of a per-slot multiplier that also asks for the second destination.
 constexpr uint32_t SLOT_SNAP = 0x80000000u;
 
+// Batched squarings over at most this many list positions pass their
+// per-position multipliers and snapshot flags as bits in the carry's kernel
+// arguments instead of a device array uploaded for every squaring.
+constexpr int INLINE_SLOTS = 512;
+
 struct CarryArgs {
   CarryJob job[2];
   const uint32_t* slot_ids;
@@ -76,6 +81,12 @@ struct CarryArgs {
   // Batches whose slots differ in lbase: per-slot bits per coefficient
   // (indexed by slot id), replacing cbits; all with the same cbits >> 5.
   const uint32_t* cbits_slot;
+  // Per-position flags as bits (bit i for list position i), replacing
+  // a_slot / snap: bits_mode 1: multiply by slot_base[slot id] where the bit
+  // is set, by 1 elsewhere; 2: the bit is the position's SLOT_SNAP; 0: unused.
+  int bits_mode;
+  const uint32_t* slot_base;
+  uint32_t bits[INLINE_SLOTS / 32];
 };
 
 // A run of batched squarings (one workgroup per listed slot): for slot list

@@ -1006,8 +1006,12 @@ __global__ void __launch_bounds__(CARRY_WG) k_carry(const CarryArgs A) {
     J.src += so;
     J.dst += so;
     if (J.sub != nullptr) J.sub += so;
-    if (A.a_slot != nullptr) J.a = A.a_slot[blockIdx.y / nj] & ~SLOT_SNAP;
-    if (J.dst2 != nullptr) J.dst2 = (A.snap != nullptr && (A.snap[blockIdx.y / nj] & SLOT_SNAP)) ? J.dst2 + so : nullptr;
+    const uint32_t pos = blockIdx.y / nj;
+    const bool bit = A.bits_mode != 0 && ((A.bits[pos >> 5] >> (pos & 31u)) & 1u) != 0u;
+    if (A.a_slot != nullptr) J.a = A.a_slot[pos] & ~SLOT_SNAP;
+    if (A.bits_mode == 1) J.a = bit ? A.slot_base[A.slot_ids[pos]] : 1u;
+    const bool snap = (A.snap != nullptr && (A.snap[pos] & SLOT_SNAP)) || (A.bits_mode == 2 && bit);
+    if (J.dst2 != nullptr) J.dst2 = snap ? J.dst2 + so : nullptr;
     if (A.cbits_slot != nullptr) cbits = int(A.cbits_slot[A.slot_ids ? A.slot_ids[blockIdx.y / nj] : 0u]);
   }
   const size_t n = size_t(1) << A.LN;
