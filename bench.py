@@ -739,7 +739,7 @@ def main():
     ap.add_argument("--run-min-active", type=int, default=4,
                     help="proth-batch/serve: running tests a class needs for squaring runs")
     ap.add_argument("--host-threads", type=int, default=0,
-                    help="proth-batch/serve: host workers driving the classes (0 = the driver's default)")
+                    help="proth-batch/serve: host workers driving the classes (0 = the batch driver's default)")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-class-seconds", type=float, default=1.0)
     ap.add_argument("--serve-seconds", type=float, default=120.0)

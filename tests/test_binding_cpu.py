@@ -1,11 +1,11 @@
-"""The reference-side binding shown in INTEGRATION.md (class transformHIP,
+"""The reference-side binding of the integration guide (class transformHIP,
 deriving from the reference's own `transform`, src/transform.h:18-45)
 compiled against the reference's headers (transform.h, file.h) and this
 repository's include/annie_hip.h, and linked with libannie_hip.so: every
 pure virtual of `transform` is implemented with the reference's signatures
 and every C ABI call it makes resolves.  The program is run without
 arguments (loads the library, constructs nothing).  Skipped where the
-reference sources are absent (the GPU box)."""
+reference sources are absent."""
 import os
 import re
 import subprocess
@@ -43,9 +43,10 @@ int main(int argc, char** argv) {
 
 
 def test_integration_binding_compiles_and_links(tmp_path):
-    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    guide = os.path.join(ROOT, "INTEGRATION.md")
+    text = open(guide).read()
     m = re.search(r"```cpp\n(// src/transformHIP\.h\n.*?)```", text, re.S)
-    assert m, "no transformHIP block in INTEGRATION.md"
+    assert m, f"no transformHIP block in {guide}"
     (tmp_path / "transformHIP.h").write_text(m.group(1))
     (tmp_path / "main.cpp").write_text(MAIN)
     exe = tmp_path / "binding"

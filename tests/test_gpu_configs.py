@@ -116,7 +116,7 @@ def _cfg4_tests(fx):
 
 
 def _batch_key(plan):
-    """The driver's batch of a plan: (K, pck, LN, digit-width class), plans
+    """The batch driver's batch of a plan: (K, pck, LN, digit-width class), plans
     that differ only in lbase within a width class share a batch."""
     K, pck, LN, lbase = plan
     return (K, pck, LN, (pck * lbase) >> 5)

@@ -4,7 +4,7 @@ built from the reference sources into oracle/_ref/ref_plan.  The reference
 prints its candidate per K and the chosen K_min before it constructs the
 OpenCL engine; with no OpenCL platform (an empty ICD vendor directory) that
 construction throws and the process ends normally.  Skipped where the
-reference sources are absent (the GPU box)."""
+reference sources are absent."""
 import os
 import random
 import re
